@@ -1,0 +1,164 @@
+/*
+ * nexd_gpu.h -- C-ABI of the MI355X time-stepping library libnexd_gpu.so.
+ *
+ * The library replaces the body of the reference's time loop
+ * (src/timeloopMod.f90:357-589) for a single partition that holds the whole
+ * mesh on one GPU.  The caller keeps the reference's set-up
+ * (src/timeloopMod.f90:135-356) and hands the library the arrays that set-up
+ * holds, in the reference's own layout: Fortran column-major, 1-based index
+ * values, REAL(8).  Every pointer argument is a plain host pointer; the
+ * library copies (and re-lays out) what it needs to device memory and owns
+ * that memory.  No call keeps a host pointer after it returns.
+ *
+ * Conventions
+ *   - every function returns 0 on success and a negative NEXD_E* code on
+ *     failure; nexd_last_error() then describes the failure;
+ *   - one context per host thread; calls on one context are not
+ *     thread-safe; independent contexts are;
+ *   - nexd_run enqueues work on the context's HIP stream and waits for it;
+ *     nexd_get_* return results that are complete.
+ *
+ * Shapes below are the Fortran shapes; "nglob" = nelem*Np.
+ */
+#ifndef NEXD_GPU_H
+#define NEXD_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NEXD_OK 0
+#define NEXD_E_ARG (-1)        /* invalid argument or inconsistent sizes */
+#define NEXD_E_HIP (-2)        /* HIP runtime failure */
+#define NEXD_E_STATE (-3)      /* call out of order (e.g. run before upload) */
+#define NEXD_E_UNSUPPORTED (-4)/* feature outside what the library implements */
+
+/* timeint values: the reference's 1 (Euler), 2 (TVD RK2), 3 (TVD RK3)
+ * (src/timeloopMod.f90:230-248) */
+#define NEXD_EULER 1
+#define NEXD_RK2 2
+#define NEXD_RK3 3
+
+/* compat bits: reproduce reference behaviour bit for bit */
+#define NEXD_COMPAT_RK3_FLOAT_WEIGHTS 1  /* 1/3, 2/3 rounded to float (src/timeloopMod.f90:129-133) */
+#define NEXD_COMPAT_PML_LAST_ELEMENT 2   /* PML ADE uses the flux of the last element
+                                            (src/rungekuttaMod.f90:57-59,80-82) */
+#define NEXD_COMPAT_ALL 3
+
+typedef struct nexd_config {
+    int nelem;             /* elements of the (single) partition */
+    int order;             /* polynomial order N, 1..5 (src/constants.h:26) */
+    int nsrc;              /* point sources in this partition */
+    int nrec;              /* receivers in this partition */
+    int nt;                /* time steps (par%nt) */
+    int timeint;           /* NEXD_EULER / NEXD_RK2 / NEXD_RK3 (par%timeint) */
+    int attenuation;       /* par%attenuation: 3 standard linear solids */
+    int pml;               /* par%set_pml */
+    int compat;            /* NEXD_COMPAT_* bits */
+    int track_displacement;/* keep ux,uy,uz for every node (movie/log); receivers
+                              are exact either way */
+    int energy;            /* compute the per-step energy of PML runs
+                              (src/timeloopMod.f90:508-516) */
+    int device;            /* HIP device ordinal */
+    double dt;             /* deltat = dtfactor*cfl (src/timeloopMod.f90:199) */
+} nexd_config;
+
+typedef struct nexd_ctx nexd_ctx;
+
+/* Creates a context and allocates device state for the configuration.
+ * Replaces the allocation block src/timeloopMod.f90:162-170. */
+int nexd_create(const nexd_config *cfg, nexd_ctx **out);
+
+/* Reference-element operators: Dr, Ds, Dt, mass (Np,Np); lift (Np,4*NpF);
+ * fmask (NpF,4).  mesh%Dr/Ds/Dt/mass/lift/fmask of typeMod (src/typeMod.f90:74-78). */
+int nexd_upload_operators(nexd_ctx *ctx, const double *dr, const double *ds, const double *dt,
+                          const double *lift, const double *mass, const int *fmask);
+
+/* Mesh: ibool (Np,nelem) must be (ie-1)*Np+j as the mesher writes it
+ * (src/meshMod.f90:796-811); rx..tz, jacobian (nglob); fscale (4*NpF,nelem);
+ * neighbor, face (4,nelem); ibt, ibn (NpF,4,nelem); nnormal, stangential,
+ * ttangential (3,4,nelem).  Partition interfaces (neighbor == -1) are not
+ * supported: the whole mesh is one partition. */
+int nexd_upload_mesh(nexd_ctx *ctx, const int *ibool,
+                     const double *rx, const double *ry, const double *rz,
+                     const double *sx, const double *sy, const double *sz,
+                     const double *tx, const double *ty, const double *tz,
+                     const double *jacobian, const double *fscale,
+                     const int *neighbor, const int *face, const int *ibt, const int *ibn,
+                     const double *nnormal, const double *stangential, const double *ttangential);
+
+/* Materials (nelem each): mesh%vpu, vsu, rho, lambdau, muu (flux matrices and
+ * elasticfluxvar, src/timeloopMod.f90:257-273) and mesh%mu, lambda (energy). */
+int nexd_upload_material(nexd_ctx *ctx, const double *vpu, const double *vsu, const double *rho,
+                         const double *lambdau, const double *muu, const double *mu,
+                         const double *lambda);
+
+/* Attenuation: mesh%wl (nMB) and anelasticvar (3*nMB*nelem)
+ * (src/timeloopMod.f90:274-282). */
+int nexd_upload_anelastic(nexd_ctx *ctx, const double *wl, const double *anelasticvar);
+
+/* PML: pmlcheck (nelem, 0/1 as C int) and the per-node profiles ddx..kz (nglob)
+ * of PMLPreCalculation (src/pmlMod.f90:90-165). */
+int nexd_upload_pml(nexd_ctx *ctx, const int *pmlcheck,
+                    const double *ddx, const double *ddy, const double *ddz,
+                    const double *alphax, const double *alphay, const double *alphaz,
+                    const double *kx, const double *ky, const double *kz);
+
+/* Sources: src%srcelem (nsrc, 1-based), src%srctype (nsrc), srcArray
+ * (Np,3,nsrc,nt) and srcArrayM (Np,6,nsrc,nt) as built at
+ * src/timeloopMod.f90:327-356.  Either array may be NULL if no source of
+ * that type exists. */
+int nexd_upload_sources(nexd_ctx *ctx, const int *srcelem, const int *srctype,
+                        const double *src_array, const double *src_array_m);
+
+/* Receivers: rec%recelem (nrec, 1-based), recInt (Np,nrec)
+ * (src/timeloopMod.f90:292-300). */
+int nexd_upload_receivers(nexd_ctx *ctx, const int *recelem, const double *rec_int);
+
+/* Resets the wavefield to the reference's initial state (every value the
+ * default-REAL literal 1e-24, i.e. 1.0000000195414813e-24 in REAL(8);
+ * PML auxiliaries 0; src/timeloopMod.f90:184-195,252-254) and the step
+ * counter to 1. */
+int nexd_reset(nexd_ctx *ctx);
+
+/* Advances steps it_begin..it_end (1-based, inclusive, it_begin must be the
+ * next step).  Records receiver rows it_begin..it_end (state before each
+ * step, src/timeloopMod.f90:358-373).  If energy is non-NULL and the
+ * context computes energy, energy[k] receives all_energy(it_begin+k). */
+int nexd_run(nexd_ctx *ctx, int it_begin, int it_end, double *energy);
+
+/* PML switch used by the caller's STA/LTA trigger (src/timeloopMod.f90:532-539). */
+int nexd_set_pml_active(nexd_ctx *ctx, int active);
+
+/* Seismograms plotux..plotaz, each (nrec,nt), rows recorded so far. */
+int nexd_get_seismograms(nexd_ctx *ctx, double *ux, double *uy, double *uz,
+                         double *vx, double *vy, double *vz,
+                         double *ax, double *ay, double *az);
+
+/* Current wavefield q (nglob,9); ux,uy,uz (nglob) if the context tracks
+ * displacement (else NEXD_E_STATE for non-NULL ux). */
+int nexd_get_state(nexd_ctx *ctx, double *q, double *ux, double *uy, double *uz);
+
+/* Device time of the last nexd_run in milliseconds (HIP events). */
+int nexd_last_run_ms(nexd_ctx *ctx, double *ms);
+
+/* Measurement only (no reference counterpart; the reference times whole
+ * steps with system_clock, src/timeloopMod.f90:544-566): launches the first
+ * RK stage kernel of one element list nrep times on the context stream
+ * between HIP events and returns the mean duration of one launch and the
+ * number of elements it advances.  list 0: elements outside the PML (all
+ * elements without PML), list 1: PML elements.  Stage 1 reads only the
+ * current state and writes the stage buffer, so repeating it leaves the
+ * state unchanged. */
+int nexd_time_stage_kernel(nexd_ctx *ctx, int list, int nrep, double *avg_ms, int *nelem_launch);
+
+void nexd_destroy(nexd_ctx *ctx);
+
+/* Description of the last failure on ctx, or of the last nexd_create
+ * failure when ctx is NULL. */
+const char *nexd_last_error(const nexd_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,692 @@
+// libnexd_gpu.so: the C-ABI of include/nexd_gpu.h over the HIP stage kernels.
+//
+// One context = one HIP stream on one device holding the whole mesh.  The
+// uploads convert the reference's column-major, 1-based arrays into the
+// device layout of plain_kernels.hpp; nexd_run replays a captured HIP graph
+// of one time step (all RK stages plus the step-end kernel) once per step.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/nexd_gpu.h"
+#include "plain_kernels.hpp"
+
+using nexd::Dims;
+using nexd::StageArgs;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+};
+
+}  // namespace
+
+struct nexd_ctx {
+    nexd_config cfg{};
+    int np = 0, npf = 0, nfp = 0, nrk = 0;
+    size_t nglob = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<void *> allocs;
+    // wavefield
+    double *q[3] = {nullptr, nullptr, nullptr};
+    double *th[2] = {nullptr, nullptr};
+    double *paux[2] = {nullptr, nullptr};
+    double *u_full = nullptr;
+    // operators and mesh
+    double *D = nullptr, *LT = nullptr, *mass = nullptr;
+    int *fmask = nullptr;
+    double *metric = nullptr, *fscale = nullptr, *frame = nullptr, *jac = nullptr;
+    int *nbr = nullptr, *ftype = nullptr;
+    double *matc = nullptr, *emat = nullptr, *anel = nullptr, *pmlc = nullptr;
+    double wl[nexd::NMB] = {0, 0, 0};
+    std::vector<int> pmlcheck;
+    int *elist_plain = nullptr, *elist_pml = nullptr;
+    int n_plain = 0, n_pml = 0;
+    bool pml_active = false;
+    // sources / receivers
+    int *src_head = nullptr, *src_next = nullptr, *srctype = nullptr;
+    double *src_array = nullptr, *src_array_m = nullptr;
+    int *recelem = nullptr, *rec_slot = nullptr, *rec_slot_of = nullptr;
+    double *rec_int = nullptr, *u_slot = nullptr, *a_slot = nullptr;
+    int nslot = 0;
+    double *seis = nullptr;
+    // per step
+    double *last_flux = nullptr, *blk_energy = nullptr, *energy = nullptr;
+    int nblk_total = 0;
+    int *d_it = nullptr;
+    int step = 0;                 // next step, 0-based
+    bool have_ops = false, have_mesh = false, have_mat = false, have_att = false, have_pml = false,
+         have_src = false, have_rec = false, initialized = false;
+    hipGraphExec_t graph = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.f;
+};
+
+namespace {
+
+int fail(nexd_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                   \
+    do {                                                                                    \
+        hipError_t _e = (expr);                                                             \
+        if (_e != hipSuccess)                                                               \
+            return fail(ctx, NEXD_E_HIP, std::string(#expr ": ") + hipGetErrorString(_e));   \
+    } while (0)
+
+template <typename T>
+int dalloc(nexd_ctx *c, T **p, size_t n) {
+    void *v = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&v, n * sizeof(T));
+    if (e != hipSuccess) return fail(c, NEXD_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->allocs.push_back(v);
+    *p = static_cast<T *>(v);
+    return 0;
+}
+
+template <typename T>
+int upload(nexd_ctx *c, T **p, const std::vector<T> &h) {
+    if (!*p) {
+        int rc = dalloc(c, p, h.size());
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+void drop_graph(nexd_ctx *c) {
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// stage launches, dispatched on the compile-time order
+// ---------------------------------------------------------------------------
+template <int ORDER>
+void launch_stage(const StageArgs<double> &a, bool att, bool pml, hipStream_t s) {
+    using Dm = Dims<ORDER>;
+    const int grid = (a.nlist + Dm::EB - 1) / Dm::EB;
+    if (grid == 0) return;
+    if (att && pml) hipLaunchKernelGGL((nexd::stage_kernel<ORDER, true, true, double>), dim3(grid), dim3(Dm::NT), 0, s, a);
+    else if (att) hipLaunchKernelGGL((nexd::stage_kernel<ORDER, true, false, double>), dim3(grid), dim3(Dm::NT), 0, s, a);
+    else if (pml) hipLaunchKernelGGL((nexd::stage_kernel<ORDER, false, true, double>), dim3(grid), dim3(Dm::NT), 0, s, a);
+    else hipLaunchKernelGGL((nexd::stage_kernel<ORDER, false, false, double>), dim3(grid), dim3(Dm::NT), 0, s, a);
+}
+
+template <int ORDER>
+int blocks_for(int n) {
+    return (n + Dims<ORDER>::EB - 1) / Dims<ORDER>::EB;
+}
+
+int nblocks(int order, int n) {
+    switch (order) {
+        case 1: return blocks_for<1>(n);
+        case 2: return blocks_for<2>(n);
+        case 3: return blocks_for<3>(n);
+        case 4: return blocks_for<4>(n);
+        default: return blocks_for<5>(n);
+    }
+}
+
+void dispatch_stage(int order, const StageArgs<double> &a, bool att, bool pml, hipStream_t s) {
+    switch (order) {
+        case 1: launch_stage<1>(a, att, pml, s); break;
+        case 2: launch_stage<2>(a, att, pml, s); break;
+        case 3: launch_stage<3>(a, att, pml, s); break;
+        case 4: launch_stage<4>(a, att, pml, s); break;
+        default: launch_stage<5>(a, att, pml, s); break;
+    }
+}
+
+template <int ORDER>
+void launch_last_flux(nexd_ctx *c, const double *qin, const double *pin) {
+    const int last = c->cfg.nelem - 1;
+    hipLaunchKernelGGL((nexd::last_flux_kernel<ORDER, double>), dim3(1), dim3(64), 0, c->stream, qin, pin, c->pmlc,
+                       c->matc, last, c->pmlcheck[last] ? 1 : 0, (int)c->nglob, c->last_flux);
+}
+
+void dispatch_last_flux(nexd_ctx *c, const double *qin, const double *pin) {
+    switch (c->cfg.order) {
+        case 1: launch_last_flux<1>(c, qin, pin); break;
+        case 2: launch_last_flux<2>(c, qin, pin); break;
+        case 3: launch_last_flux<3>(c, qin, pin); break;
+        case 4: launch_last_flux<4>(c, qin, pin); break;
+        default: launch_last_flux<5>(c, qin, pin); break;
+    }
+}
+
+template <int ORDER>
+void launch_step_end(nexd_ctx *c, int row_offset, int advance) {
+    const double *be = (c->cfg.energy && c->pml_active && c->have_pml) ? c->blk_energy : nullptr;
+    hipLaunchKernelGGL((nexd::step_end_kernel<ORDER, double>), dim3(1), dim3(256), 0, c->stream, c->q[0], c->u_slot,
+                       c->a_slot, c->rec_int, c->recelem, c->rec_slot_of, c->cfg.nrec, (int)c->nglob, c->cfg.nt,
+                       row_offset, advance, c->seis, be, c->nblk_total, c->energy, c->d_it);
+}
+
+void dispatch_step_end(nexd_ctx *c, int row_offset, int advance) {
+    switch (c->cfg.order) {
+        case 1: launch_step_end<1>(c, row_offset, advance); break;
+        case 2: launch_step_end<2>(c, row_offset, advance); break;
+        case 3: launch_step_end<3>(c, row_offset, advance); break;
+        case 4: launch_step_end<4>(c, row_offset, advance); break;
+        default: launch_step_end<5>(c, row_offset, advance); break;
+    }
+}
+
+StageArgs<double> base_args(nexd_ctx *c) {
+    const bool f32w = (c->cfg.compat & NEXD_COMPAT_RK3_FLOAT_WEIGHTS) != 0;
+    StageArgs<double> a{};
+    a.nglob = (int)c->nglob;
+    a.scheme = c->cfg.timeint;
+    a.dt = c->cfg.dt;
+    a.w13 = f32w ? (double)(1.0f / 3.0f) : 1.0 / 3.0;
+    a.w23 = f32w ? (double)(2.0f / 3.0f) : 2.0 / 3.0;
+    a.pml_compat = (c->cfg.compat & NEXD_COMPAT_PML_LAST_ELEMENT) ? 1 : 0;
+    a.last_flux = c->last_flux;
+    a.D = c->D; a.LT = c->LT; a.fmask = c->fmask;
+    a.metric = c->metric; a.fscale = c->fscale; a.frame = c->frame; a.matc = c->matc;
+    a.nbr = c->nbr; a.ftype = c->ftype; a.anel = c->anel;
+    for (int i = 0; i < nexd::NMB; ++i) a.wl[i] = c->wl[i];
+    a.pmlc = c->pmlc;
+    a.src_head = c->src_head; a.src_next = c->src_next; a.srctype = c->srctype;
+    a.src_array = c->src_array; a.src_array_m = c->src_array_m; a.nsrc = c->cfg.nsrc;
+    a.d_it = c->d_it;
+    a.rec_slot = c->rec_slot; a.u_slot = c->u_slot; a.a_slot = c->a_slot; a.u_full = c->u_full;
+    a.mass = c->mass; a.jac = c->jac; a.emat = c->emat;
+    return a;
+}
+
+// enqueue one full time step on the context stream
+void enqueue_step(nexd_ctx *c) {
+    const int scheme = c->cfg.timeint;
+    const bool att = c->cfg.attenuation != 0;
+    const bool pml = c->have_pml && c->pml_active && c->n_pml > 0;
+    StageArgs<double> a = base_args(c);
+    const int nrk = c->nrk;
+    for (int irk = 1; irk <= nrk; ++irk) {
+        double *qin, *qbase = c->q[0], *qout;
+        double *thin, *thout, *pin, *pout;
+        if (scheme == NEXD_EULER) {
+            qin = c->q[0]; qout = c->q[1]; thin = c->th[0]; thout = c->th[0]; pin = c->paux[0]; pout = c->paux[0];
+        } else if (scheme == NEXD_RK2) {
+            if (irk == 1) { qin = c->q[0]; qout = c->q[1]; thin = c->th[0]; thout = c->th[1]; pin = c->paux[0]; pout = c->paux[1]; }
+            else { qin = c->q[1]; qout = c->q[0]; thin = c->th[1]; thout = c->th[0]; pin = c->paux[1]; pout = c->paux[0]; }
+        } else {
+            if (irk == 1) { qin = c->q[0]; qout = c->q[1]; thin = c->th[0]; thout = c->th[1]; pin = c->paux[0]; pout = c->paux[1]; }
+            else if (irk == 2) { qin = c->q[1]; qout = c->q[2]; thin = c->th[1]; thout = c->th[1]; pin = c->paux[1]; pout = c->paux[1]; }
+            else { qin = c->q[2]; qout = c->q[0]; thin = c->th[1]; thout = c->th[0]; pin = c->paux[1]; pout = c->paux[0]; }
+        }
+        a.irk = irk;
+        a.last_stage = irk == nrk ? 1 : 0;
+        a.qin = qin; a.qbase = qbase; a.qout = qout;
+        a.thin = thin; a.thbase = c->th[0]; a.thout = thout;
+        a.pin = pin; a.pbase = c->paux[0]; a.pout = pout;
+        if (pml && a.pml_compat) dispatch_last_flux(c, qin, pin);
+        const bool energy = a.last_stage && c->cfg.energy && pml;
+        // elements without PML (every element when PML is off)
+        a.elist = c->elist_plain;
+        a.nlist = c->n_plain;
+        a.blk_energy = energy ? c->blk_energy : nullptr;
+        dispatch_stage(c->cfg.order, a, att, false, c->stream);
+        if (pml) {
+            a.elist = c->elist_pml;
+            a.nlist = c->n_pml;
+            a.blk_energy = energy ? c->blk_energy + nblocks(c->cfg.order, c->n_plain) : nullptr;
+            dispatch_stage(c->cfg.order, a, att, true, c->stream);
+        }
+        if (scheme == NEXD_EULER)
+            (void)hipMemcpyAsync(c->q[0], c->q[1], sizeof(double) * 9 * c->nglob, hipMemcpyDeviceToDevice, c->stream);
+    }
+    dispatch_step_end(c, 1, 1);
+}
+
+int rebuild_lists(nexd_ctx *c) {
+    // elements without PML first (all elements when PML is inactive), PML ones second
+    std::vector<int> plain, pmll;
+    const bool pml = c->have_pml && c->pml_active;
+    for (int e = 0; e < c->cfg.nelem; ++e) {
+        if (pml && c->pmlcheck[e]) pmll.push_back(e);
+        else if (pml) plain.push_back(e);
+    }
+    if (!pml) {
+        plain.resize(c->cfg.nelem);
+        for (int e = 0; e < c->cfg.nelem; ++e) plain[e] = e;
+    }
+    c->n_plain = (int)plain.size();
+    c->n_pml = (int)pmll.size();
+    if (c->n_plain) {
+        HIPCHK(c, hipMemcpy(c->elist_plain, plain.data(), sizeof(int) * plain.size(), hipMemcpyHostToDevice));
+    }
+    if (c->n_pml) {
+        HIPCHK(c, hipMemcpy(c->elist_pml, pmll.data(), sizeof(int) * pmll.size(), hipMemcpyHostToDevice));
+    }
+    c->nblk_total = nblocks(c->cfg.order, c->n_plain) + nblocks(c->cfg.order, c->n_pml);
+    drop_graph(c);
+    return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+int nexd_create(const nexd_config *cfg, nexd_ctx **out) {
+    if (!cfg || !out) return fail(nullptr, NEXD_E_ARG, "null argument");
+    *out = nullptr;
+    if (cfg->order < 1 || cfg->order > 5) return fail(nullptr, NEXD_E_UNSUPPORTED, "order must be 1..5");
+    if (cfg->nelem <= 0 || cfg->nt <= 0 || cfg->nsrc < 0 || cfg->nrec < 0)
+        return fail(nullptr, NEXD_E_ARG, "nelem, nt must be positive; nsrc, nrec non-negative");
+    if (cfg->timeint < 1 || cfg->timeint > 3) return fail(nullptr, NEXD_E_UNSUPPORTED, "timeint must be 1, 2 or 3");
+    if (!(cfg->dt > 0)) return fail(nullptr, NEXD_E_ARG, "dt must be positive");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(nullptr, NEXD_E_HIP, "no HIP device");
+    if (cfg->device < 0 || cfg->device >= ndev) return fail(nullptr, NEXD_E_ARG, "device ordinal out of range");
+    nexd_ctx *c = new nexd_ctx();
+    c->cfg = *cfg;
+    c->np = (cfg->order + 1) * (cfg->order + 2) * (cfg->order + 3) / 6;
+    c->npf = (cfg->order + 1) * (cfg->order + 2) / 2;
+    c->nfp = 4 * c->npf;
+    c->nglob = (size_t)cfg->nelem * c->np;
+    c->nrk = cfg->timeint == 1 ? 1 : (cfg->timeint == 2 ? 2 : 3);
+    auto bail = [&](int rc) { g_create_error = c->err; nexd_destroy(c); return rc; };
+    if (hipSetDevice(cfg->device) != hipSuccess) return bail(fail(c, NEXD_E_HIP, "hipSetDevice failed"));
+    if (hipStreamCreate(&c->stream) != hipSuccess) return bail(fail(c, NEXD_E_HIP, "hipStreamCreate failed"));
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+        return bail(fail(c, NEXD_E_HIP, "hipEventCreate failed"));
+    const size_t ng = c->nglob;
+    int rc = 0;
+    for (int i = 0; i < 3 && !rc; ++i) rc = dalloc(c, &c->q[i], 9 * ng);
+    if (!rc && cfg->attenuation) (rc = dalloc(c, &c->th[0], 18 * ng)) || (rc = dalloc(c, &c->th[1], 18 * ng));
+    if (!rc && cfg->pml)
+        (rc = dalloc(c, &c->paux[0], 27 * ng)) || (rc = dalloc(c, &c->paux[1], 27 * ng)) ||
+            (rc = dalloc(c, &c->pmlc, 9 * ng)) || (rc = dalloc(c, &c->last_flux, 27 * c->np));
+    if (!rc && cfg->track_displacement) rc = dalloc(c, &c->u_full, 3 * ng);
+    if (!rc)
+        (rc = dalloc(c, &c->elist_plain, cfg->nelem)) || (rc = dalloc(c, &c->elist_pml, cfg->nelem)) ||
+            (rc = dalloc(c, &c->d_it, 1)) ||
+            (rc = dalloc(c, &c->seis, (size_t)9 * cfg->nt * (cfg->nrec > 0 ? cfg->nrec : 1))) ||
+            (rc = dalloc(c, &c->energy, cfg->nt)) || (rc = dalloc(c, &c->rec_slot, cfg->nelem)) ||
+            (rc = dalloc(c, &c->src_head, cfg->nelem)) || (rc = dalloc(c, &c->blk_energy, (size_t)cfg->nelem + 64));
+    if (rc) return bail(rc);
+    c->pmlcheck.assign(cfg->nelem, 0);
+    c->pml_active = cfg->pml != 0;
+    // defaults: no sources, no receivers
+    std::vector<int> minus(cfg->nelem, -1);
+    if ((rc = upload(c, &c->src_head, minus)) || (rc = upload(c, &c->rec_slot, minus))) return bail(rc);
+    if (!cfg->attenuation) c->have_att = true;
+    if (!cfg->pml) c->have_pml = false;
+    if (cfg->nsrc == 0) c->have_src = true;
+    if (cfg->nrec == 0) c->have_rec = true;
+    if ((rc = rebuild_lists(c))) return bail(rc);
+    *out = c;
+    return NEXD_OK;
+}
+
+int nexd_upload_operators(nexd_ctx *c, const double *dr, const double *ds, const double *dt, const double *lift,
+                          const double *mass, const int *fmask) {
+    if (!c || !dr || !ds || !dt || !lift || !mass || !fmask) return fail(c, NEXD_E_ARG, "null operator");
+    const int np = c->np, npf = c->npf, nfp = c->nfp;
+    std::vector<double> D(3 * np * np), LT(nfp * np), M(np * np);
+    const double *src[3] = {dr, ds, dt};
+    for (int d = 0; d < 3; ++d)
+        for (int i = 0; i < np; ++i)
+            for (int m = 0; m < np; ++m) D[(d * np + i) * np + m] = src[d][(size_t)m * np + i];
+    for (int k = 0; k < nfp; ++k)
+        for (int j = 0; j < np; ++j) LT[k * np + j] = lift[(size_t)k * np + j];
+    for (int i = 0; i < np; ++i)
+        for (int k = 0; k < np; ++k) M[i * np + k] = mass[(size_t)k * np + i];
+    std::vector<int> fm(4 * npf);
+    for (int f = 0; f < 4; ++f)
+        for (int i = 0; i < npf; ++i) {
+            int v = fmask[f * npf + i] - 1;
+            if (v < 0 || v >= np) return fail(c, NEXD_E_ARG, "fmask entry out of range");
+            fm[f * npf + i] = v;
+        }
+    int rc;
+    if ((rc = upload(c, &c->D, D)) || (rc = upload(c, &c->LT, LT)) || (rc = upload(c, &c->mass, M)) ||
+        (rc = upload(c, &c->fmask, fm)))
+        return rc;
+    c->have_ops = true;
+    drop_graph(c);
+    return NEXD_OK;
+}
+
+int nexd_upload_mesh(nexd_ctx *c, const int *ibool, const double *rx, const double *ry, const double *rz,
+                     const double *sx, const double *sy, const double *sz, const double *tx, const double *ty,
+                     const double *tz, const double *jacobian, const double *fscale, const int *neighbor,
+                     const int *face, const int *ibt, const int *ibn, const double *nnormal,
+                     const double *stangential, const double *ttangential) {
+    if (!c || !ibool || !rx || !ry || !rz || !sx || !sy || !sz || !tx || !ty || !tz || !jacobian || !fscale ||
+        !neighbor || !face || !ibt || !ibn || !nnormal || !stangential || !ttangential)
+        return fail(c, NEXD_E_ARG, "null mesh array");
+    if (!c->have_ops) return fail(c, NEXD_E_STATE, "upload operators before the mesh");
+    const int np = c->np, npf = c->npf, nfp = c->nfp, nel = c->cfg.nelem;
+    const size_t ng = c->nglob;
+    for (size_t i = 0; i < ng; ++i)
+        if (ibool[i] != (int)i + 1) return fail(c, NEXD_E_UNSUPPORTED, "ibool must be (ie-1)*Np+j");
+    std::vector<int> fm(4 * npf);
+    HIPCHK(c, hipMemcpy(fm.data(), c->fmask, sizeof(int) * fm.size(), hipMemcpyDeviceToHost));
+    std::vector<double> met(9 * ng);
+    const double *ms[9] = {rx, ry, rz, sx, sy, sz, tx, ty, tz};
+    for (int k = 0; k < 9; ++k) std::memcpy(met.data() + k * ng, ms[k], sizeof(double) * ng);
+    std::vector<double> jac(jacobian, jacobian + ng);
+    std::vector<double> fs(fscale, fscale + (size_t)nel * nfp);
+    std::vector<double> fr((size_t)nel * 36);
+    std::vector<int> nb((size_t)nel * 4 * npf), ft((size_t)nel * 4);
+    for (int e = 0; e < nel; ++e) {
+        for (int f = 0; f < 4; ++f) {
+            for (int k = 0; k < 3; ++k) {
+                fr[(e * 4 + f) * 9 + k] = nnormal[(e * 4 + f) * 3 + k];
+                fr[(e * 4 + f) * 9 + 3 + k] = stangential[(e * 4 + f) * 3 + k];
+                fr[(e * 4 + f) * 9 + 6 + k] = ttangential[(e * 4 + f) * 3 + k];
+            }
+            const int n = neighbor[e * 4 + f], fc = face[e * 4 + f];
+            for (int i = 0; i < npf; ++i) {
+                const size_t idx = ((size_t)e * 4 + f) * npf + i;
+                if ((size_t)ibt[idx] != (size_t)e * np + fm[f * npf + i] + 1)
+                    return fail(c, NEXD_E_UNSUPPORTED, "ibt must be ibool(fmask(:,is),ie)");
+            }
+            if (n > 0) {
+                if (fc < 1 || fc > 4) return fail(c, NEXD_E_ARG, "face of an interior face must be 1..4");
+                ft[e * 4 + f] = fc - 1;
+                for (int i = 0; i < npf; ++i) {
+                    const size_t idx = ((size_t)e * 4 + f) * npf + i;
+                    const int v = ibn[idx] - 1;
+                    if (v < 0 || (size_t)v >= ng) return fail(c, NEXD_E_ARG, "ibn out of range");
+                    nb[idx] = v;
+                }
+            } else if (n == 0) {
+                if (fc != -1 && fc != -2) return fail(c, NEXD_E_ARG, "boundary face must be free (-1) or absorbing (-2)");
+                ft[e * 4 + f] = fc;
+                for (int i = 0; i < npf; ++i) nb[((size_t)e * 4 + f) * npf + i] = -1;
+            } else {
+                return fail(c, NEXD_E_UNSUPPORTED, "partition interfaces (neighbor == -1) are not supported");
+            }
+        }
+    }
+    int rc;
+    if ((rc = upload(c, &c->metric, met)) || (rc = upload(c, &c->jac, jac)) || (rc = upload(c, &c->fscale, fs)) ||
+        (rc = upload(c, &c->frame, fr)) || (rc = upload(c, &c->nbr, nb)) || (rc = upload(c, &c->ftype, ft)))
+        return rc;
+    c->have_mesh = true;
+    drop_graph(c);
+    return NEXD_OK;
+}
+
+int nexd_upload_material(nexd_ctx *c, const double *vpu, const double *vsu, const double *rho, const double *lambdau,
+                         const double *muu, const double *mu, const double *lambda) {
+    if (!c || !vpu || !vsu || !rho || !lambdau || !muu || !mu || !lambda) return fail(c, NEXD_E_ARG, "null material");
+    const int nel = c->cfg.nelem;
+    std::vector<double> mc((size_t)nel * 8, 0.0), em((size_t)nel * 2);
+    for (int e = 0; e < nel; ++e) {
+        if (!(rho[e] > 0)) return fail(c, NEXD_E_ARG, "density must be positive");
+        mc[e * 8 + 0] = vpu[e]; mc[e * 8 + 1] = vsu[e]; mc[e * 8 + 2] = rho[e];
+        mc[e * 8 + 3] = lambdau[e]; mc[e * 8 + 4] = muu[e];
+        em[e * 2] = mu[e]; em[e * 2 + 1] = lambda[e];
+    }
+    int rc;
+    if ((rc = upload(c, &c->matc, mc)) || (rc = upload(c, &c->emat, em))) return rc;
+    c->have_mat = true;
+    drop_graph(c);
+    return NEXD_OK;
+}
+
+int nexd_upload_anelastic(nexd_ctx *c, const double *wl, const double *anelasticvar) {
+    if (!c || !wl || !anelasticvar) return fail(c, NEXD_E_ARG, "null anelastic array");
+    if (!c->cfg.attenuation) return fail(c, NEXD_E_STATE, "context created without attenuation");
+    for (int i = 0; i < nexd::NMB; ++i) c->wl[i] = wl[i];
+    std::vector<double> av(anelasticvar, anelasticvar + (size_t)c->cfg.nelem * nexd::NMB * 3);
+    int rc = upload(c, &c->anel, av);
+    if (rc) return rc;
+    c->have_att = true;
+    drop_graph(c);
+    return NEXD_OK;
+}
+
+int nexd_upload_pml(nexd_ctx *c, const int *pmlcheck, const double *ddx, const double *ddy, const double *ddz,
+                    const double *alphax, const double *alphay, const double *alphaz, const double *kx,
+                    const double *ky, const double *kz) {
+    if (!c || !pmlcheck || !ddx || !ddy || !ddz || !alphax || !alphay || !alphaz || !kx || !ky || !kz)
+        return fail(c, NEXD_E_ARG, "null PML array");
+    if (!c->cfg.pml) return fail(c, NEXD_E_STATE, "context created without PML");
+    const size_t ng = c->nglob;
+    std::vector<double> pc(9 * ng);
+    for (size_t g = 0; g < ng; ++g) {
+        pc[0 * ng + g] = alphax[g]; pc[1 * ng + g] = alphay[g]; pc[2 * ng + g] = alphaz[g];
+        pc[3 * ng + g] = ddx[g] / kx[g]; pc[4 * ng + g] = ddy[g] / ky[g]; pc[5 * ng + g] = ddz[g] / kz[g];
+        pc[6 * ng + g] = kx[g]; pc[7 * ng + g] = ky[g]; pc[8 * ng + g] = kz[g];
+    }
+    for (int e = 0; e < c->cfg.nelem; ++e) c->pmlcheck[e] = pmlcheck[e] ? 1 : 0;
+    int rc = upload(c, &c->pmlc, pc);
+    if (rc) return rc;
+    c->have_pml = true;
+    return rebuild_lists(c);
+}
+
+int nexd_upload_sources(nexd_ctx *c, const int *srcelem, const int *srctype, const double *src_array,
+                        const double *src_array_m) {
+    if (!c) return fail(c, NEXD_E_ARG, "null context");
+    const int ns = c->cfg.nsrc, np = c->np, nel = c->cfg.nelem;
+    if (ns == 0) { c->have_src = true; return NEXD_OK; }
+    if (!srcelem || !srctype) return fail(c, NEXD_E_ARG, "null source array");
+    std::vector<int> head(nel, -1), next(ns, -1), st(ns);
+    bool any_f = false, any_m = false;
+    for (int s = ns - 1; s >= 0; --s) {
+        const int e = srcelem[s] - 1;
+        if (e < 0 || e >= nel) return fail(c, NEXD_E_ARG, "srcelem out of range");
+        if (srctype[s] != 0 && srctype[s] != 1) return fail(c, NEXD_E_ARG, "srctype must be 0 or 1");
+        st[s] = srctype[s];
+        any_f |= srctype[s] == 0;
+        any_m |= srctype[s] == 1;
+        next[s] = head[e];
+        head[e] = s;
+    }
+    if ((any_f && !src_array) || (any_m && !src_array_m)) return fail(c, NEXD_E_ARG, "missing source array");
+    const size_t n3 = (size_t)np * 3 * ns * c->cfg.nt, n6 = (size_t)np * 6 * ns * c->cfg.nt;
+    std::vector<double> a3 = src_array ? std::vector<double>(src_array, src_array + n3) : std::vector<double>(1, 0.0);
+    std::vector<double> a6 = src_array_m ? std::vector<double>(src_array_m, src_array_m + n6) : std::vector<double>(1, 0.0);
+    int rc;
+    if ((rc = upload(c, &c->src_head, head)) || (rc = upload(c, &c->src_next, next)) ||
+        (rc = upload(c, &c->srctype, st)) || (rc = upload(c, &c->src_array, a3)) ||
+        (rc = upload(c, &c->src_array_m, a6)))
+        return rc;
+    c->have_src = true;
+    drop_graph(c);
+    return NEXD_OK;
+}
+
+int nexd_upload_receivers(nexd_ctx *c, const int *recelem, const double *rec_int) {
+    if (!c) return fail(c, NEXD_E_ARG, "null context");
+    const int nr = c->cfg.nrec, np = c->np, nel = c->cfg.nelem;
+    if (nr == 0) { c->have_rec = true; return NEXD_OK; }
+    if (!recelem || !rec_int) return fail(c, NEXD_E_ARG, "null receiver array");
+    std::vector<int> slot_of_elem(nel, -1), slot_of_rec(nr), re(nr);
+    int nslot = 0;
+    for (int r = 0; r < nr; ++r) {
+        const int e = recelem[r] - 1;
+        if (e < 0 || e >= nel) return fail(c, NEXD_E_ARG, "recelem out of range");
+        if (slot_of_elem[e] < 0) slot_of_elem[e] = nslot++;
+        slot_of_rec[r] = slot_of_elem[e];
+        re[r] = e;
+    }
+    std::vector<double> w(rec_int, rec_int + (size_t)np * nr);
+    int rc;
+    if (!c->u_slot) {
+        if ((rc = dalloc(c, &c->u_slot, (size_t)nslot * 3 * np)) || (rc = dalloc(c, &c->a_slot, (size_t)nslot * 3 * np)))
+            return rc;
+    }
+    c->nslot = nslot;
+    if ((rc = upload(c, &c->rec_slot, slot_of_elem)) || (rc = upload(c, &c->rec_slot_of, slot_of_rec)) ||
+        (rc = upload(c, &c->recelem, re)) || (rc = upload(c, &c->rec_int, w)))
+        return rc;
+    c->have_rec = true;
+    drop_graph(c);
+    return NEXD_OK;
+}
+
+int nexd_reset(nexd_ctx *c) {
+    if (!c) return fail(c, NEXD_E_ARG, "null context");
+    const size_t ng = c->nglob;
+    // the reference assigns the default-REAL literal 1e-24 (src/timeloopMod.f90:184-195)
+    const double tiny_v = (double)1e-24f;
+    std::vector<double> tiny(9 * ng, tiny_v);
+    for (int i = 0; i < 3; ++i) HIPCHK(c, hipMemcpy(c->q[i], tiny.data(), sizeof(double) * 9 * ng, hipMemcpyHostToDevice));
+    if (c->cfg.attenuation) {
+        std::vector<double> t18(18 * ng, tiny_v);
+        for (int i = 0; i < 2; ++i) HIPCHK(c, hipMemcpy(c->th[i], t18.data(), sizeof(double) * 18 * ng, hipMemcpyHostToDevice));
+    }
+    if (c->cfg.pml) {
+        std::vector<double> z(27 * ng, 0.0);
+        for (int i = 0; i < 2; ++i) HIPCHK(c, hipMemcpy(c->paux[i], z.data(), sizeof(double) * 27 * ng, hipMemcpyHostToDevice));
+    }
+    if (c->u_full) HIPCHK(c, hipMemcpy(c->u_full, tiny.data(), sizeof(double) * 3 * ng, hipMemcpyHostToDevice));
+    if (c->nslot) {
+        std::vector<double> t(3 * (size_t)c->nslot * c->np, tiny_v);
+        HIPCHK(c, hipMemcpy(c->u_slot, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->a_slot, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice));
+    }
+    std::vector<double> zs((size_t)9 * c->cfg.nt * (c->cfg.nrec > 0 ? c->cfg.nrec : 1), 0.0);
+    HIPCHK(c, hipMemcpy(c->seis, zs.data(), sizeof(double) * zs.size(), hipMemcpyHostToDevice));
+    std::vector<double> ze(c->cfg.nt, 0.0);
+    HIPCHK(c, hipMemcpy(c->energy, ze.data(), sizeof(double) * ze.size(), hipMemcpyHostToDevice));
+    int zero = 0;
+    HIPCHK(c, hipMemcpy(c->d_it, &zero, sizeof(int), hipMemcpyHostToDevice));
+    c->step = 0;
+    c->initialized = true;
+    return NEXD_OK;
+}
+
+int nexd_run(nexd_ctx *c, int it_begin, int it_end, double *energy) {
+    if (!c) return fail(c, NEXD_E_ARG, "null context");
+    if (!c->have_ops || !c->have_mesh || !c->have_mat || !c->have_att || (c->cfg.pml && !c->have_pml) ||
+        !c->have_src || !c->have_rec)
+        return fail(c, NEXD_E_STATE, "upload operators, mesh, material, anelastic/PML, sources and receivers first");
+    if (!c->initialized) {
+        int rc = nexd_reset(c);
+        if (rc) return rc;
+    }
+    if (it_begin != c->step + 1) return fail(c, NEXD_E_ARG, "it_begin must be the next step");
+    if (it_end < it_begin || it_end > c->cfg.nt) return fail(c, NEXD_E_ARG, "it_end out of range");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->step == 0) dispatch_step_end(c, 0, 0);   // row 1: the initial state
+    if (!c->graph) {
+        hipGraph_t g = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_step(c);
+        HIPCHK(c, hipStreamEndCapture(c->stream, &g));
+        hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) { c->graph = nullptr; return fail(c, NEXD_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)); }
+    }
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int it = it_begin; it <= it_end; ++it) HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+    if (energy && c->cfg.energy && c->cfg.pml)
+        HIPCHK(c, hipMemcpy(energy, c->energy + (it_begin - 1), sizeof(double) * (it_end - it_begin + 1), hipMemcpyDeviceToHost));
+    c->step = it_end;
+    return NEXD_OK;
+}
+
+int nexd_set_pml_active(nexd_ctx *c, int active) {
+    if (!c) return fail(c, NEXD_E_ARG, "null context");
+    c->pml_active = active != 0;
+    return rebuild_lists(c);
+}
+
+int nexd_get_seismograms(nexd_ctx *c, double *ux, double *uy, double *uz, double *vx, double *vy, double *vz,
+                         double *ax, double *ay, double *az) {
+    if (!c) return fail(c, NEXD_E_ARG, "null context");
+    double *outs[9] = {ux, uy, uz, vx, vy, vz, ax, ay, az};
+    const size_t n = (size_t)c->cfg.nt * c->cfg.nrec;
+    for (int k = 0; k < 9; ++k) {
+        if (!outs[k]) continue;
+        HIPCHK(c, hipMemcpy(outs[k], c->seis + k * n, sizeof(double) * n, hipMemcpyDeviceToHost));
+    }
+    return NEXD_OK;
+}
+
+int nexd_get_state(nexd_ctx *c, double *q, double *ux, double *uy, double *uz) {
+    if (!c) return fail(c, NEXD_E_ARG, "null context");
+    const size_t ng = c->nglob;
+    if (q) HIPCHK(c, hipMemcpy(q, c->q[0], sizeof(double) * 9 * ng, hipMemcpyDeviceToHost));
+    double *u[3] = {ux, uy, uz};
+    for (int k = 0; k < 3; ++k) {
+        if (!u[k]) continue;
+        if (!c->u_full) return fail(c, NEXD_E_STATE, "context does not track displacement");
+        HIPCHK(c, hipMemcpy(u[k], c->u_full + k * ng, sizeof(double) * ng, hipMemcpyDeviceToHost));
+    }
+    return NEXD_OK;
+}
+
+int nexd_last_run_ms(nexd_ctx *c, double *ms) {
+    if (!c || !ms) return fail(c, NEXD_E_ARG, "null argument");
+    *ms = c->last_ms;
+    return NEXD_OK;
+}
+
+int nexd_time_stage_kernel(nexd_ctx *c, int list, int nrep, double *avg_ms, int *nelem_launch) {
+    if (!c || !avg_ms || nrep <= 0 || (list != 0 && list != 1)) return fail(c, NEXD_E_ARG, "bad argument");
+    if (!c->initialized) return fail(c, NEXD_E_STATE, "run or reset before timing");
+    const bool pml = c->have_pml && c->pml_active && c->n_pml > 0;
+    if (list == 1 && !pml) return fail(c, NEXD_E_STATE, "no PML element list");
+    StageArgs<double> a = base_args(c);
+    a.irk = 1;
+    a.last_stage = c->nrk == 1 ? 1 : 0;
+    a.qin = c->q[0]; a.qbase = c->q[0]; a.qout = c->q[1];
+    a.thin = c->th[0]; a.thbase = c->th[0]; a.thout = c->th[1];
+    a.pin = c->paux[0]; a.pbase = c->paux[0]; a.pout = c->paux[1];
+    if (c->nrk == 1) { a.thout = c->th[1] ? c->th[1] : c->th[0]; a.pout = c->paux[1] ? c->paux[1] : c->paux[0]; a.u_full = nullptr; a.rec_slot = nullptr; }
+    a.blk_energy = nullptr;
+    a.elist = list == 0 ? c->elist_plain : c->elist_pml;
+    a.nlist = list == 0 ? c->n_plain : c->n_pml;
+    if (a.last_stage) return fail(c, NEXD_E_UNSUPPORTED, "timing needs a multi-stage scheme");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (list == 1 && a.pml_compat) dispatch_last_flux(c, a.qin, a.pin);
+    dispatch_stage(c->cfg.order, a, c->cfg.attenuation != 0, list == 1, c->stream);   // warm-up
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int r = 0; r < nrep; ++r) dispatch_stage(c->cfg.order, a, c->cfg.attenuation != 0, list == 1, c->stream);
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipGetLastError());
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *avg_ms = ms / nrep;
+    if (nelem_launch) *nelem_launch = a.nlist;
+    return NEXD_OK;
+}
+
+void nexd_destroy(nexd_ctx *c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
+    for (void *p : c->allocs) (void)hipFree(p);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *nexd_last_error(const nexd_ctx *c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,591 @@
+// Stage kernels of the single-shot time loop ("plain" path, FP64 or FP32).
+//
+// One kernel launch advances one Runge-Kutta stage for a list of elements:
+// it fuses what the reference does in its element loop and its separate RK
+// pass (src/timeloopMod.f90:395-480, src/rungekuttaMod.f90:27-120):
+//   volume term      derivatives of the fluxes through Dr/Ds/Dt, metric terms
+//                    applied per node (src/timeloopMod.f90:407-420)
+//   face term        upwind flux T*(A+|A|)*T^-1 applied factored to the jump
+//                    (src/waveMod.f90:269-336), lifted through LIFT
+//                    (src/timeloopMod.f90:463-465)
+//   attenuation      memory-variable RHS and the stress coupling
+//                    (src/timeloopMod.f90:426-461)
+//   PML              flux modification and the ADE update of f',g',h'
+//                    (src/timeloopMod.f90:399-405, src/rungekuttaMod.f90:52-63)
+//   sources          srcArray/srcArrayM of the step (src/timeloopMod.f90:467-475)
+//   RK update        into a separate output buffer (double buffering replaces
+//                    the reference's qm/qn copies, src/timeloopMod.f90:376-391)
+//   post-step        receiver-element displacement/acceleration and the energy
+//                    partials on the last stage (src/timeloopMod.f90:482-516)
+//
+// Layout in HBM: fields are structure-of-arrays, element-major within a field
+// (the reference's q(nglob,9) column-major), so the Np nodes of an element are
+// contiguous in every field and a block of consecutive elements is one
+// contiguous run per field.  Threads map to (element-in-block, node).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace nexd {
+
+template <int ORDER>
+struct Dims {
+    static constexpr int NP = (ORDER + 1) * (ORDER + 2) * (ORDER + 3) / 6;
+    static constexpr int NPF = (ORDER + 1) * (ORDER + 2) / 2;
+    static constexpr int NFP = 4 * NPF;
+    static constexpr int EB = (256 / NP) > 0 ? (256 / NP) : 1;   // elements per block
+    static constexpr int NT = EB * NP;                          // threads per block
+};
+
+constexpr int NMB = 3;
+
+template <typename R>
+struct StageArgs {
+    int nglob;
+    const int *elist;           // elements handled by this launch
+    int nlist;
+    int irk;                    // 1-based stage
+    int scheme;                 // 1 Euler, 2 RK2, 3 RK3
+    int last_stage;             // 1 on the stage that completes a step
+    R dt;
+    R w13, w23;                 // RK3 final-stage weights
+    const R *qin;               // [9][nglob] stage input
+    const R *qbase;             // [9][nglob] step-start state (RK base)
+    R *qout;                    // [9][nglob]
+    const R *thin;              // [18][nglob]
+    const R *thbase;
+    R *thout;
+    const R *pin;               // [27][nglob] PML f',g',h'
+    const R *pbase;
+    R *pout;
+    const R *last_flux;         // [27][NP] fluxes of the last element (compat)
+    int pml_compat;
+    const R *D;                 // [3][NP][NP]  D[d][i][m]
+    const R *LT;                // [NFP][NP]    LIFT(j,k) at LT[k][j]
+    const int *fmask;           // [4][NPF] 0-based
+    const R *metric;            // [9][nglob] rx ry rz sx sy sz tx ty tz
+    const R *fscale;            // [nelem][NFP]
+    const R *frame;             // [nelem][4][9] n s t
+    const R *matc;              // [nelem][8] vp vs rho lam mu
+    const int *nbr;             // [nelem][4][NPF] neighbour node or -1
+    const int *ftype;           // [nelem][4] >=0 interior, -1 free, -2 absorbing
+    const R *anel;              // [nelem][NMB][3]
+    R wl[NMB];
+    const R *pmlc;              // [9][nglob] alpha xyz, d/k xyz, k xyz
+    const int *src_head;        // [nelem] first source of the element or -1
+    const int *src_next;        // [nsrc]
+    const int *srctype;         // [nsrc]
+    const R *src_array;         // [nt][nsrc][3][NP]
+    const R *src_array_m;       // [nt][nsrc][6][NP]
+    int nsrc;
+    const int *d_it;            // current step (0-based), device-resident
+    const int *rec_slot;        // [nelem] receiver-element slot or -1
+    R *u_slot;                  // [nslot][3][NP]
+    R *a_slot;                  // [nslot][3][NP]
+    R *u_full;                  // [3][nglob] or null
+    R *blk_energy;              // [gridDim] or null
+    const R *mass;              // [NP][NP]
+    const R *jac;               // [nglob]
+    const R *emat;              // [nelem][2] mu, lambda for the energy
+};
+
+// ---------------------------------------------------------------------------
+// upwind flux of one face node: out = T (A+|A|) T^-1 dq, the operator the
+// reference assembles as VT (src/timeloopMod.f90:262-271); free surfaces
+// apply diag(-2,0,0,-2,0,-2,0,0,0) between A+|A| and T^-1.
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ void upwind_flux(const R *fr, R vp, R vs, R rho, R lam, R mu, bool free,
+                                            const R dq[9], R out[9], R aout[6], bool att) {
+    const R n0 = fr[0], n1 = fr[1], n2 = fr[2];
+    const R s0 = fr[3], s1 = fr[4], s2 = fr[5];
+    const R t0 = fr[6], t1 = fr[7], t2 = fr[8];
+    // rows 1, 4, 6, 7, 8, 9 of T^-1 dq (the others meet zero columns of A+|A|)
+    R w1 = n0 * n0 * dq[0] + n1 * n1 * dq[1] + n2 * n2 * dq[2] + 2 * n1 * n0 * dq[3] + 2 * n2 * n1 * dq[4] +
+           2 * n2 * n0 * dq[5];
+    R w4 = n0 * s0 * dq[0] + n1 * s1 * dq[1] + n2 * s2 * dq[2] + (n1 * s0 + n0 * s1) * dq[3] +
+           (n2 * s1 + n1 * s2) * dq[4] + (n2 * s0 + n0 * s2) * dq[5];
+    R w6 = n0 * t0 * dq[0] + n1 * t1 * dq[1] + n2 * t2 * dq[2] + (n1 * t0 + n0 * t1) * dq[3] +
+           (n2 * t1 + n1 * t2) * dq[4] + (n2 * t0 + n0 * t2) * dq[5];
+    R w7 = n0 * dq[6] + n1 * dq[7] + n2 * dq[8];
+    R w8 = s0 * dq[6] + s1 * dq[7] + s2 * dq[8];
+    R w9 = t0 * dq[6] + t1 * dq[7] + t2 * dq[8];
+    if (free) {
+        w1 = -2 * w1; w4 = -2 * w4; w6 = -2 * w6;
+        w7 = 0; w8 = 0; w9 = 0;
+    }
+    const R l2m = lam + 2 * mu;
+    const R z1 = vp * w1 - l2m * w7;
+    const R z2 = (lam * vp) / l2m * w1 - lam * w7;   // z3 == z2
+    const R z4 = vs * w4 - mu * w8;
+    const R z6 = vs * w6 - mu * w9;
+    const R irho = R(1) / rho;
+    const R z7 = -irho * w1 + vp * w7;
+    const R z8 = -irho * w4 + vs * w8;
+    const R z9 = -irho * w6 + vs * w9;
+    // T z (z5 == 0)
+    out[0] = n0 * n0 * z1 + (s0 * s0 + t0 * t0) * z2 + 2 * n0 * s0 * z4 + 2 * n0 * t0 * z6;
+    out[1] = n1 * n1 * z1 + (s1 * s1 + t1 * t1) * z2 + 2 * n1 * s1 * z4 + 2 * n1 * t1 * z6;
+    out[2] = n2 * n2 * z1 + (s2 * s2 + t2 * t2) * z2 + 2 * n2 * s2 * z4 + 2 * n2 * t2 * z6;
+    out[3] = n1 * n0 * z1 + (s1 * s0 + t1 * t0) * z2 + (n1 * s0 + n0 * s1) * z4 + (n1 * t0 + n0 * t1) * z6;
+    out[4] = n2 * n1 * z1 + (s2 * s1 + t2 * t1) * z2 + (n2 * s1 + n1 * s2) * z4 + (n2 * t1 + n1 * t2) * z6;
+    out[5] = n2 * n0 * z1 + (s2 * s0 + t2 * t0) * z2 + (n2 * s0 + n0 * s2) * z4 + (n2 * t0 + n0 * t2) * z6;
+    out[6] = n0 * z7 + s0 * z8 + t0 * z9;
+    out[7] = n1 * z7 + s1 * z8 + t1 * z9;
+    out[8] = n2 * z7 + s2 * z8 + t2 * z9;
+    if (att) {
+        // anelastic part: T(1:6,1:6) * A_anel * T^-1 dq
+        // (src/waveMod.f90:146-159, src/timeloopMod.f90:270-271)
+        const R a1 = w1 / (vp * rho) - w7;
+        const R a4 = w4 / (2 * vs * rho) - w8 / 2;
+        const R a6 = w6 / (2 * vs * rho) - w9 / 2;
+        aout[0] = n0 * n0 * a1 + 2 * n0 * s0 * a4 + 2 * n0 * t0 * a6;
+        aout[1] = n1 * n1 * a1 + 2 * n1 * s1 * a4 + 2 * n1 * t1 * a6;
+        aout[2] = n2 * n2 * a1 + 2 * n2 * s2 * a4 + 2 * n2 * t2 * a6;
+        aout[3] = n1 * n0 * a1 + (n1 * s0 + n0 * s1) * a4 + (n1 * t0 + n0 * t1) * a6;
+        aout[4] = n2 * n1 * a1 + (n2 * s1 + n1 * s2) * a4 + (n2 * t1 + n1 * t2) * a6;
+        aout[5] = n2 * n0 * a1 + (n2 * s0 + n0 * s2) * a4 + (n2 * t0 + n0 * t2) * a6;
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ R rk_combine(int scheme, int irk, R a, R base, R rhs, R dt, R w13, R w23) {
+    // src/rungekuttaMod.f90:90-120
+    if (irk == 1) return a + rhs * dt;
+    if (scheme == 2) return R(0.5) * (base + a + rhs * dt);
+    if (irk == 2) return R(0.75) * base + R(0.25) * (a + rhs * dt);
+    return w13 * base + w23 * (a + rhs * dt);
+}
+
+// elastic fluxes F,G,H of one node (src/waveMod.f90:29-63), 9 columns each
+template <typename R>
+__device__ __forceinline__ void elastic_flux_node(const R q[9], R ef0, R ef1, R ef2, R ef3, R F[9], R G[9], R H[9]) {
+    F[0] = ef0 * q[6]; F[1] = ef1 * q[6]; F[2] = ef1 * q[6]; F[3] = ef2 * q[7]; F[4] = 0;
+    F[5] = ef2 * q[8]; F[6] = ef3 * q[0]; F[7] = ef3 * q[3]; F[8] = ef3 * q[5];
+    G[0] = ef1 * q[7]; G[1] = ef0 * q[7]; G[2] = ef1 * q[7]; G[3] = ef2 * q[6]; G[4] = ef2 * q[8];
+    G[5] = 0; G[6] = ef3 * q[3]; G[7] = ef3 * q[1]; G[8] = ef3 * q[4];
+    H[0] = ef1 * q[8]; H[1] = ef1 * q[8]; H[2] = ef0 * q[8]; H[3] = 0; H[4] = ef2 * q[7];
+    H[5] = ef2 * q[6]; H[6] = ef3 * q[5]; H[7] = ef3 * q[4]; H[8] = ef3 * q[2];
+}
+
+// flux columns kept for PML elements: F without column 5, G without 6,
+// H without 4 (those are identically zero, src/waveMod.f90:37-62)
+__device__ constexpr int kPmlCols = 24;
+
+template <int ORDER, bool ATT, bool PML, typename R>
+__global__ void __launch_bounds__(Dims<ORDER>::NT)
+stage_kernel(StageArgs<R> a) {
+    using Dm = Dims<ORDER>;
+    constexpr int NP = Dm::NP, NPF = Dm::NPF, NFP = Dm::NFP, EB = Dm::EB, NT = Dm::NT;
+    constexpr int NFC = ATT ? 15 : 9;                    // face values per face node
+    constexpr int XSZ = PML ? kPmlCols * NT : 0;
+    constexpr int FSZ = EB * NFP * NFC;
+    constexpr int USZ = (XSZ > FSZ ? XSZ : FSZ) > 15 * NT ? (XSZ > FSZ ? XSZ : FSZ) : 15 * NT;
+    __shared__ R sQ[9 * NT];
+    __shared__ R sU[USZ];                                // PML flux columns / face values / energy
+    __shared__ R sRed[NT];
+
+    const int t = threadIdx.x;
+    const int eb = t / NP, j = t - eb * NP;
+    const int idx = blockIdx.x * EB + eb;
+    const bool valid = idx < a.nlist;
+    const int e = valid ? a.elist[idx] : 0;
+    const size_t ng = (size_t)a.nglob;
+    const size_t g = (size_t)e * NP + j;
+
+    R qs[9];
+#pragma unroll
+    for (int f = 0; f < 9; ++f) {
+        qs[f] = valid ? a.qin[f * ng + g] : R(0);
+        sQ[f * NT + t] = qs[f];
+    }
+    R ef0 = 0, ef1 = 0, ef2 = 0, ef3 = 0, vp = 0, vs = 0, rho = 1, lam = 0, mu = 0;
+    if (valid) {
+        const R *mc = a.matc + (size_t)e * 8;
+        vp = mc[0]; vs = mc[1]; rho = mc[2]; lam = mc[3]; mu = mc[4];
+        ef0 = -(lam + 2 * mu); ef1 = -lam; ef2 = -mu; ef3 = R(-1.0) / rho;
+    }
+    // PML: modified fluxes (src/timeloopMod.f90:399-405)
+    R Fm[9], Gm[9], Hm[9], pcur[27];
+    if constexpr (PML) {
+        elastic_flux_node(qs, ef0, ef1, ef2, ef3, Fm, Gm, Hm);
+        R kx = 1, ky = 1, kz = 1;
+        if (valid) {
+            kx = a.pmlc[6 * ng + g]; ky = a.pmlc[7 * ng + g]; kz = a.pmlc[8 * ng + g];
+        }
+#pragma unroll
+        for (int i = 0; i < 27; ++i) pcur[i] = valid ? a.pin[i * ng + g] : R(0);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            Fm[i] = (pcur[i] + Fm[i]) / kx;
+            Gm[i] = (pcur[9 + i] + Gm[i]) / ky;
+            Hm[i] = (pcur[18 + i] + Hm[i]) / kz;
+        }
+        int c = 0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) if (i != 4) sU[(c++) * NT + t] = Fm[i];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) if (i != 5) sU[(c++) * NT + t] = Gm[i];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) if (i != 3) sU[(c++) * NT + t] = Hm[i];
+    }
+    __syncthreads();
+
+    // ---- volume term ------------------------------------------------------
+    R met[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) met[k] = valid ? a.metric[k * ng + g] : R(0);
+    const R rx = met[0], ry = met[1], rz = met[2], sx = met[3], sy = met[4], sz = met[5];
+    const R tx = met[6], ty = met[7], tz = met[8];
+    const R *Dr = a.D + j * NP, *Ds = a.D + NP * NP + j * NP, *Dt = a.D + 2 * NP * NP + j * NP;
+    const int ebase = eb * NP;
+    R rq[9];
+    R gv[9];   // velocity gradients d(v_c)/d(x_k), c-major: gv[c*3+k]
+    if constexpr (!PML) {
+        R dr[9], ds[9], dt[9];
+#pragma unroll
+        for (int f = 0; f < 9; ++f) { dr[f] = 0; ds[f] = 0; dt[f] = 0; }
+        for (int m = 0; m < NP; ++m) {
+            const R cr = Dr[m], cs = Ds[m], ct = Dt[m];
+#pragma unroll
+            for (int f = 0; f < 9; ++f) {
+                const R v = sQ[f * NT + ebase + m];
+                dr[f] += cr * v; ds[f] += cs * v; dt[f] += ct * v;
+            }
+        }
+        R dx[9], dy[9], dz[9];
+#pragma unroll
+        for (int f = 0; f < 9; ++f) {
+            dx[f] = rx * dr[f] + sx * ds[f] + tx * dt[f];
+            dy[f] = ry * dr[f] + sy * ds[f] + ty * dt[f];
+            dz[f] = rz * dr[f] + sz * ds[f] + tz * dt[f];
+        }
+        rq[0] = ef0 * dx[6] + ef1 * dy[7] + ef1 * dz[8];
+        rq[1] = ef1 * dx[6] + ef0 * dy[7] + ef1 * dz[8];
+        rq[2] = ef1 * dx[6] + ef1 * dy[7] + ef0 * dz[8];
+        rq[3] = ef2 * (dx[7] + dy[6]);
+        rq[4] = ef2 * (dy[8] + dz[7]);
+        rq[5] = ef2 * (dx[8] + dz[6]);
+        rq[6] = ef3 * (dx[0] + dy[3] + dz[5]);
+        rq[7] = ef3 * (dx[3] + dy[1] + dz[4]);
+        rq[8] = ef3 * (dx[5] + dy[4] + dz[2]);
+        if constexpr (ATT) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) { gv[c * 3 + 0] = dx[6 + c]; gv[c * 3 + 1] = dy[6 + c]; gv[c * 3 + 2] = dz[6 + c]; }
+        }
+    } else {
+        // 24 flux columns through Dr, Ds, Dt (src/timeloopMod.f90:407-420)
+        R vF[9], vG[9], vH[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) { vF[i] = 0; vG[i] = 0; vH[i] = 0; }
+#pragma unroll
+        for (int c = 0; c < kPmlCols; ++c) {
+            R sr = 0, ss = 0, st = 0;
+            const R *col = sU + c * NT + ebase;
+            for (int m = 0; m < NP; ++m) {
+                const R v = col[m];
+                sr += Dr[m] * v; ss += Ds[m] * v; st += Dt[m] * v;
+            }
+            if (c < 8) {
+                const int i = c < 4 ? c : c + 1;
+                vF[i] = rx * sr + sx * ss + tx * st;
+            } else if (c < 16) {
+                const int i = (c - 8) < 5 ? (c - 8) : (c - 8) + 1;
+                vG[i] = ry * sr + sy * ss + ty * st;
+            } else {
+                const int i = (c - 16) < 3 ? (c - 16) : (c - 16) + 1;
+                vH[i] = rz * sr + sz * ss + tz * st;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) rq[i] = vF[i] + vG[i] + vH[i];
+        if constexpr (ATT) {
+            R dr3[3] = {0, 0, 0}, ds3[3] = {0, 0, 0}, dt3[3] = {0, 0, 0};
+            for (int m = 0; m < NP; ++m) {
+                const R cr = Dr[m], cs = Ds[m], ct = Dt[m];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const R v = sQ[(6 + c) * NT + ebase + m];
+                    dr3[c] += cr * v; ds3[c] += cs * v; dt3[c] += ct * v;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                gv[c * 3 + 0] = rx * dr3[c] + sx * ds3[c] + tx * dt3[c];
+                gv[c * 3 + 1] = ry * dr3[c] + sy * ds3[c] + ty * dt3[c];
+                gv[c * 3 + 2] = rz * dr3[c] + sz * ds3[c] + tz * dt3[c];
+            }
+        }
+    }
+    __syncthreads();   // sU is reused for face values below
+
+    // ---- face term: upwind flux per face node into sU ----------------------
+    for (int it = t; it < EB * NFP; it += NT) {
+        const int fb = it / NFP, k = it - fb * NFP;
+        const int fidx = blockIdx.x * EB + fb;
+        if (fidx >= a.nlist) continue;
+        const int fe = a.elist[fidx];
+        const int fi = k / NPF, i = k - fi * NPF;
+        const int own = fb * NP + a.fmask[fi * NPF + i];
+        const int ft = a.ftype[fe * 4 + fi];
+        R dq[9];
+        if (ft >= 0) {
+            const size_t gn = (size_t)a.nbr[((size_t)fe * 4 + fi) * NPF + i];
+#pragma unroll
+            for (int f = 0; f < 9; ++f) dq[f] = a.qin[f * ng + gn] - sQ[f * NT + own];
+        } else {
+#pragma unroll
+            for (int f = 0; f < 9; ++f) dq[f] = sQ[f * NT + own];
+        }
+        const R *mc = a.matc + (size_t)fe * 8;
+        R fl[9], af[6];
+        upwind_flux<R>(a.frame + ((size_t)fe * 4 + fi) * 9, mc[0], mc[1], mc[2], mc[3], mc[4], ft == -1, dq, fl, af, ATT);
+        const R sgn = ft == -2 ? R(-1) : R(1);
+        const R fs = a.fscale[(size_t)fe * NFP + k];
+        R *dst = sU + (fb * NFP + k) * NFC;
+#pragma unroll
+        for (int f = 0; f < 9; ++f) dst[f] = fs * (sgn * fl[f]) / 2;
+        if constexpr (ATT) {
+#pragma unroll
+            for (int f = 0; f < 6; ++f) dst[9 + f] = fs * (sgn * af[f]) / 2;
+        }
+    }
+    __syncthreads();
+
+    // ---- LIFT --------------------------------------------------------------
+    R lf[NFC];
+#pragma unroll
+    for (int f = 0; f < NFC; ++f) lf[f] = 0;
+    {
+        const R *src = sU + eb * NFP * NFC;
+        for (int k = 0; k < NFP; ++k) {
+            const R l = a.LT[k * NP + j];
+#pragma unroll
+            for (int f = 0; f < NFC; ++f) lf[f] += l * src[k * NFC + f];
+        }
+    }
+
+    const int it_step = *a.d_it;
+    R rhs[9];
+#pragma unroll
+    for (int f = 0; f < 9; ++f) rhs[f] = rq[f];
+
+    // ---- attenuation (src/timeloopMod.f90:426-461) --------------------------
+    R thm[ATT ? 18 : 1], arq[ATT ? 18 : 1];
+    if constexpr (ATT) {
+#pragma unroll
+        for (int c = 0; c < 18; ++c) thm[c] = valid ? a.thin[c * ng + g] : R(0);
+        // strain-rate components of the anelastic flux divergence
+        const R ex = gv[0], ey = gv[4], ez = gv[8];
+        const R exy = gv[3] + gv[1];   // d vy/dx + d vx/dy
+        const R eyz = gv[7] + gv[5];   // d vz/dy + d vy/dz
+        const R exz = gv[6] + gv[2];   // d vz/dx + d vx/dz
+#pragma unroll
+        for (int i = 0; i < NMB; ++i) {
+            const R w = a.wl[i];
+            const R vol[6] = {-w * ex, -w * ey, -w * ez, R(-0.5) * w * exy, R(-0.5) * w * eyz, R(-0.5) * w * exz};
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                arq[i * 6 + c] = vol[c] + w * lf[9 + c] - w * thm[i * 6 + c];
+        }
+        const R *av = a.anel + (size_t)e * NMB * 3;
+#pragma unroll
+        for (int i = 0; i < NMB; ++i) {
+            const R a1 = valid ? av[i * 3 + 0] : R(0), a2 = valid ? av[i * 3 + 1] : R(0), a3 = valid ? av[i * 3 + 2] : R(0);
+            const R *th = thm + i * 6;
+            rhs[0] = rhs[0] + a3 * th[0] + a1 * (th[1] + th[2]);
+            rhs[1] = rhs[1] + a3 * th[1] + a1 * (th[2] + th[0]);
+            rhs[2] = rhs[2] + a3 * th[2] + a1 * (th[0] + th[1]);
+            rhs[3] = rhs[3] + a2 * th[3];
+            rhs[4] = rhs[4] + a2 * th[4];
+            rhs[5] = rhs[5] + a2 * th[5];
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < 9; ++f) rhs[f] = rhs[f] + lf[f];
+
+    // ---- sources (src/timeloopMod.f90:467-475) -----------------------------
+    if (valid) {
+        for (int s = a.src_head[e]; s >= 0; s = a.src_next[s]) {
+            if (a.srctype[s] == 0) {
+                const R *sp = a.src_array + (((size_t)it_step * a.nsrc + s) * 3) * NP + j;
+                rhs[6] += sp[0]; rhs[7] += sp[NP]; rhs[8] += sp[2 * NP];
+            } else {
+                const R *sp = a.src_array_m + (((size_t)it_step * a.nsrc + s) * 6) * NP + j;
+#pragma unroll
+                for (int c = 0; c < 6; ++c) rhs[c] += sp[c * NP];
+            }
+        }
+    }
+
+    // ---- Runge-Kutta update -----------------------------------------------
+    R qnew[9];
+#pragma unroll
+    for (int f = 0; f < 9; ++f) {
+        const R base = (a.irk == 1 || !valid) ? qs[f] : a.qbase[f * ng + g];
+        qnew[f] = rk_combine<R>(a.scheme, a.irk, qs[f], base, rhs[f], a.dt, a.w13, a.w23);
+        if (valid) a.qout[f * ng + g] = qnew[f];
+    }
+    if constexpr (ATT) {
+#pragma unroll
+        for (int c = 0; c < 18; ++c) {
+            const R base = (a.irk == 1 || !valid) ? thm[c] : a.thbase[c * ng + g];
+            const R v = rk_combine<R>(a.scheme, a.irk, thm[c], base, arq[c], a.dt, a.w13, a.w23);
+            if (valid) a.thout[c * ng + g] = v;
+        }
+    }
+    if constexpr (PML) {
+        // ADE update of the auxiliary fluxes (src/rungekuttaMod.f90:52-63)
+        R al[3] = {1, 1, 1}, dk[3] = {0, 0, 0};
+        if (valid) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) { al[d] = a.pmlc[d * ng + g]; dk[d] = a.pmlc[(3 + d) * ng + g]; }
+        }
+#pragma unroll
+        for (int c = 0; c < 27; ++c) {
+            const int d = c / 9, i = c - d * 9;
+            R use;
+            if (a.pml_compat) use = a.last_flux[c * NP + j];
+            else use = d == 0 ? Fm[i] : (d == 1 ? Gm[i] : Hm[i]);
+            const R cur = pcur[c];
+            const R dadt = -al[d] * cur - dk[d] * (cur + use);
+            const R base = (a.irk == 1 || !valid) ? cur : a.pbase[c * ng + g];
+            const R v = rk_combine<R>(a.scheme, a.irk, cur, base, dadt, a.dt, a.w13, a.w23);
+            if (valid) a.pout[c * ng + g] = v;
+        }
+    }
+
+    // ---- post-step (src/timeloopMod.f90:482-516) ---------------------------
+    if (a.last_stage) {
+        if (valid) {
+            if (a.u_full) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) a.u_full[c * ng + g] = a.u_full[c * ng + g] + a.dt * qnew[6 + c];
+            }
+            const int slot = a.rec_slot[e];
+            if (slot >= 0) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    R *u = a.u_slot + ((size_t)slot * 3 + c) * NP + j;
+                    *u = *u + a.dt * qnew[6 + c];
+                    a.a_slot[((size_t)slot * 3 + c) * NP + j] = rhs[6 + c];
+                }
+            }
+        }
+        if (a.blk_energy) {
+            // energy partial of this block: 0.5*J*(rho v.Mv + sigma.M e) per element
+            __syncthreads();
+            R lam_e = 0, mu_e = 0;
+            if (valid) { mu_e = a.emat[(size_t)e * 2]; lam_e = a.emat[(size_t)e * 2 + 1]; }
+            const R temp1 = R(1.) / (4 * mu_e * mu_e + 6 * mu_e * lam_e);
+            const R temp2 = 2 * mu_e;
+            R y[15];
+#pragma unroll
+            for (int c = 0; c < 9; ++c) y[c] = qnew[c];
+            y[9] = (2 * (lam_e + mu_e) * qnew[0] - lam_e * (qnew[1] + qnew[2])) / temp1;
+            y[10] = (2 * (lam_e + mu_e) * qnew[1] - lam_e * (qnew[0] + qnew[2])) / temp1;
+            y[11] = (2 * (lam_e + mu_e) * qnew[2] - lam_e * (qnew[0] + qnew[0])) / temp1;
+            y[12] = qnew[3] / temp2; y[13] = qnew[4] / temp2; y[14] = qnew[5] / temp2;
+#pragma unroll
+            for (int c = 0; c < 15; ++c) sU[c * NT + t] = valid ? y[c] : R(0);
+            __syncthreads();
+            // node j's share: x_j * (M y)_j for the 9 quadratic forms
+            R mv[9];
+#pragma unroll
+            for (int c = 0; c < 9; ++c) mv[c] = 0;
+            for (int k = 0; k < NP; ++k) {
+                const R mjk = a.mass[j * NP + k];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) mv[6 + c] += mjk * sU[(6 + c) * NT + ebase + k];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) mv[c] += mjk * sU[(9 + c) * NT + ebase + k];
+            }
+            R part = 0;
+            if (valid) {
+                const R jac0 = a.jac[(size_t)e * NP];
+                const R d = rho * (qnew[6] * mv[6]) + rho * (qnew[7] * mv[7]) + rho * (qnew[8] * mv[8]) +
+                            qnew[0] * mv[0] + qnew[1] * mv[1] + qnew[2] * mv[2] +
+                            2 * (qnew[3] * mv[3] + qnew[4] * mv[4] + qnew[5] * mv[5]);
+                part = R(0.5) * jac0 * d;
+            }
+            sRed[t] = part;
+            __syncthreads();
+            if (t == 0) {
+                R acc = 0;
+                for (int k = 0; k < NT; ++k) acc += sRed[k];
+                a.blk_energy[blockIdx.x] = acc;
+            }
+        }
+    }
+}
+
+// Fluxes of the last element for the PML ADE in compat mode (H1): one
+// thread per node of element nelem-1.
+template <int ORDER, typename R>
+__global__ void last_flux_kernel(const R *qin, const R *pin, const R *pmlc, const R *matc, int last, int last_is_pml,
+                                 int nglob, R *out) {
+    constexpr int NP = Dims<ORDER>::NP;
+    const int j = threadIdx.x;
+    if (j >= NP) return;
+    const size_t ng = (size_t)nglob, g = (size_t)last * NP + j;
+    R q[9];
+#pragma unroll
+    for (int f = 0; f < 9; ++f) q[f] = qin[f * ng + g];
+    const R *mc = matc + (size_t)last * 8;
+    const R lam = mc[3], mu = mc[4], rho = mc[2];
+    R F[9], G[9], H[9];
+    elastic_flux_node<R>(q, -(lam + 2 * mu), -lam, -mu, R(-1.0) / rho, F, G, H);
+    if (last_is_pml) {
+        const R kx = pmlc[6 * ng + g], ky = pmlc[7 * ng + g], kz = pmlc[8 * ng + g];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            F[i] = (pin[i * ng + g] + F[i]) / kx;
+            G[i] = (pin[(9 + i) * ng + g] + G[i]) / ky;
+            H[i] = (pin[(18 + i) * ng + g] + H[i]) / kz;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        out[i * NP + j] = F[i];
+        out[(9 + i) * NP + j] = G[i];
+        out[(18 + i) * NP + j] = H[i];
+    }
+}
+
+// Receiver sampling (src/timeloopMod.f90:358-373) of row `row0 + *d_it`,
+// energy reduction of the step, and the step-counter advance.  One block.
+template <int ORDER, typename R>
+__global__ void step_end_kernel(const R *q, const R *u_slot, const R *a_slot, const R *rec_int, const int *recelem,
+                                const int *rec_slot_of, int nrec, int nglob, int nt, int row_offset, int advance,
+                                R *seis, const R *blk_energy, int nblk, R *energy_out, int *d_it) {
+    constexpr int NP = Dims<ORDER>::NP;
+    __shared__ int s_it;
+    if (threadIdx.x == 0) s_it = *d_it;
+    __syncthreads();
+    const int it = s_it;
+    const int row = it + row_offset;
+    if (row < nt) {
+        for (int w = threadIdx.x; w < nrec * 9; w += blockDim.x) {
+            const int r = w / 9, k = w - r * 9;
+            const int e = recelem[r], slot = rec_slot_of[r];
+            const R *wt = rec_int + (size_t)r * NP;
+            R acc = 0;
+            for (int jn = 0; jn < NP; ++jn) {
+                R v;
+                if (k < 3) v = u_slot[((size_t)slot * 3 + k) * NP + jn];
+                else if (k < 6) v = q[(size_t)(6 + k - 3) * nglob + (size_t)e * NP + jn];
+                else v = a_slot[((size_t)slot * 3 + (k - 6)) * NP + jn];
+                acc = acc + v * wt[jn];
+            }
+            seis[((size_t)k * nt + row) * nrec + r] = acc;
+        }
+    }
+    if (advance && blk_energy && threadIdx.x == 0) {
+        R acc = 0;
+        for (int b = 0; b < nblk; ++b) acc += blk_energy[b];
+        energy_out[it] = acc;
+    }
+    __syncthreads();
+    if (advance && threadIdx.x == 0) *d_it = it + 1;
+}
+
+}  // namespace nexd

@@ -179,12 +179,14 @@ int oracle_run(const oracle_problem *p, int nsteps, double *seismo, double *ener
     double *fp = NULL, *fpn = NULL, *fpm = NULL, *epml = NULL;
     elem_mats *mats = malloc(sizeof(elem_mats) * (size_t)nelem);
     if (!q || !qn || !qm || !rq || !ux || !ax || !mats) return -2;
-    for (size_t i = 0; i < 9 * nglob; ++i) { q[i] = 1e-24; rq[i] = 1e-24; qn[i] = 1e-24; }
-    for (size_t i = 0; i < 3 * nglob; ++i) { ux[i] = 1e-24; ax[i] = 1e-24; }
+    /* the default-REAL literal 1e-24 widened to REAL(8) (src/timeloopMod.f90:184-195) */
+    const double tiny = (double)1e-24f;
+    for (size_t i = 0; i < 9 * nglob; ++i) { q[i] = tiny; rq[i] = tiny; qn[i] = tiny; }
+    for (size_t i = 0; i < 3 * nglob; ++i) { ux[i] = tiny; ax[i] = tiny; }
     if (p->attenuation) {
         th = malloc(sizeof(double) * nth); thn = malloc(sizeof(double) * nth);
         thm = malloc(sizeof(double) * nth); arq = malloc(sizeof(double) * nth);
-        for (size_t i = 0; i < nth; ++i) { th[i] = 1e-24; arq[i] = 1e-24; thn[i] = 0.0; }
+        for (size_t i = 0; i < nth; ++i) { th[i] = tiny; arq[i] = tiny; thn[i] = 0.0; }
     }
     if (p->pml) {
         fp = calloc(27 * nglob, sizeof(double)); fpn = calloc(27 * nglob, sizeof(double));

@@ -1,0 +1,40 @@
+"""The C-ABI library loads and exports every entry point of include/*.h
+(no GPU work: runs on the CPU-only host too)."""
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+from nexd_amd import capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names.update(re.findall(r"\b(nexd\w*)\s*\(", text))
+    return sorted(names)
+
+
+def test_headers_declare_entry_points():
+    names = _declared()
+    assert "nexd_create" in names and "nexd_run" in names
+    assert set(capi.EXPORTS) <= set(names)
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(capi.LIB_PATH):
+        pytest.fail("libnexd_gpu.so is not built; run __graft_entry__.build()")
+    lib = ctypes.CDLL(capi.LIB_PATH)
+    missing = [n for n in _declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_last_error_without_context_is_a_string():
+    lib = capi.load()
+    assert isinstance(lib.nexd_last_error(None), bytes)
