@@ -118,29 +118,39 @@ void drop_graph(nexd_ctx *c) {
 // ---------------------------------------------------------------------------
 // stage launches, dispatched on the compile-time order
 // ---------------------------------------------------------------------------
+template <int ORDER, bool ATT, bool PML>
+void launch_one(const StageArgs<double> &a, hipStream_t s) {
+    using S = nexd::Shape<ORDER, ATT, PML>;
+    const int grid = (a.nlist + S::EB - 1) / S::EB;
+    if (grid == 0) return;
+    hipLaunchKernelGGL((nexd::stage_kernel<ORDER, ATT, PML, double>), dim3(grid), dim3(S::NTHR), 0, s, a);
+}
+
 template <int ORDER>
 void launch_stage(const StageArgs<double> &a, bool att, bool pml, hipStream_t s) {
-    using Dm = Dims<ORDER>;
-    const int grid = (a.nlist + Dm::EB - 1) / Dm::EB;
-    if (grid == 0) return;
-    if (att && pml) hipLaunchKernelGGL((nexd::stage_kernel<ORDER, true, true, double>), dim3(grid), dim3(Dm::NT), 0, s, a);
-    else if (att) hipLaunchKernelGGL((nexd::stage_kernel<ORDER, true, false, double>), dim3(grid), dim3(Dm::NT), 0, s, a);
-    else if (pml) hipLaunchKernelGGL((nexd::stage_kernel<ORDER, false, true, double>), dim3(grid), dim3(Dm::NT), 0, s, a);
-    else hipLaunchKernelGGL((nexd::stage_kernel<ORDER, false, false, double>), dim3(grid), dim3(Dm::NT), 0, s, a);
+    if (att && pml) launch_one<ORDER, true, true>(a, s);
+    else if (att) launch_one<ORDER, true, false>(a, s);
+    else if (pml) launch_one<ORDER, false, true>(a, s);
+    else launch_one<ORDER, false, false>(a, s);
 }
 
 template <int ORDER>
-int blocks_for(int n) {
-    return (n + Dims<ORDER>::EB - 1) / Dims<ORDER>::EB;
+int blocks_for(int n, bool att, bool pml) {
+    int eb;
+    if (att && pml) eb = nexd::Shape<ORDER, true, true>::EB;
+    else if (att) eb = nexd::Shape<ORDER, true, false>::EB;
+    else if (pml) eb = nexd::Shape<ORDER, false, true>::EB;
+    else eb = nexd::Shape<ORDER, false, false>::EB;
+    return (n + eb - 1) / eb;
 }
 
-int nblocks(int order, int n) {
+int nblocks(int order, int n, bool att, bool pml) {
     switch (order) {
-        case 1: return blocks_for<1>(n);
-        case 2: return blocks_for<2>(n);
-        case 3: return blocks_for<3>(n);
-        case 4: return blocks_for<4>(n);
-        default: return blocks_for<5>(n);
+        case 1: return blocks_for<1>(n, att, pml);
+        case 2: return blocks_for<2>(n, att, pml);
+        case 3: return blocks_for<3>(n, att, pml);
+        case 4: return blocks_for<4>(n, att, pml);
+        default: return blocks_for<5>(n, att, pml);
     }
 }
 
@@ -174,7 +184,7 @@ void dispatch_last_flux(nexd_ctx *c, const double *qin, const double *pin) {
 template <int ORDER>
 void launch_step_end(nexd_ctx *c, int row_offset, int advance) {
     const double *be = (c->cfg.energy && c->pml_active && c->have_pml) ? c->blk_energy : nullptr;
-    hipLaunchKernelGGL((nexd::step_end_kernel<ORDER, double>), dim3(1), dim3(256), 0, c->stream, c->q[0], c->u_slot,
+    hipLaunchKernelGGL((nexd::step_end_kernel<ORDER, double>), dim3(1), dim3(1024), 0, c->stream, c->q[0], c->u_slot,
                        c->a_slot, c->rec_int, c->recelem, c->rec_slot_of, c->cfg.nrec, (int)c->nglob, c->cfg.nt,
                        row_offset, advance, c->seis, be, c->nblk_total, c->energy, c->d_it);
 }
@@ -247,7 +257,7 @@ void enqueue_step(nexd_ctx *c) {
         if (pml) {
             a.elist = c->elist_pml;
             a.nlist = c->n_pml;
-            a.blk_energy = energy ? c->blk_energy + nblocks(c->cfg.order, c->n_plain) : nullptr;
+            a.blk_energy = energy ? c->blk_energy + nblocks(c->cfg.order, c->n_plain, att, false) : nullptr;
             dispatch_stage(c->cfg.order, a, att, true, c->stream);
         }
         if (scheme == NEXD_EULER)
@@ -276,7 +286,8 @@ int rebuild_lists(nexd_ctx *c) {
     if (c->n_pml) {
         HIPCHK(c, hipMemcpy(c->elist_pml, pmll.data(), sizeof(int) * pmll.size(), hipMemcpyHostToDevice));
     }
-    c->nblk_total = nblocks(c->cfg.order, c->n_plain) + nblocks(c->cfg.order, c->n_pml);
+    const bool att = c->cfg.attenuation != 0;
+    c->nblk_total = nblocks(c->cfg.order, c->n_plain, att, false) + nblocks(c->cfg.order, c->n_pml, att, true);
     drop_graph(c);
     return 0;
 }

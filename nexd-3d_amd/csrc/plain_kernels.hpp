@@ -171,158 +171,130 @@ __device__ __forceinline__ void elastic_flux_node(const R q[9], R ef0, R ef1, R 
 // H without 4 (those are identically zero, src/waveMod.f90:37-62)
 __device__ constexpr int kPmlCols = 24;
 
+// Block shape of one stage kernel instantiation.  Threads play two roles:
+// node threads (element-in-block, node) for loads, the face term, LIFT and
+// the update; column threads (element-in-block, column) for the volume
+// term, where one column of q (or of the PML-modified fluxes) stays in
+// registers while the rows of Dr/Ds/Dt stream through scalar loads.
+template <int ORDER, bool ATT, bool PML>
+struct Shape {
+    static constexpr int NP = Dims<ORDER>::NP, NPF = Dims<ORDER>::NPF, NFP = Dims<ORDER>::NFP;
+    static constexpr int NCOL = PML ? (ATT ? 27 : 24) : 9;     // columns per element
+    static constexpr int NV = PML ? (ATT ? 33 : 24) : 27;      // volume values per node
+    static constexpr int NFC = ATT ? 15 : 9;                    // face values per face node
+    static constexpr int NFCP = ATT ? 16 : 10;                  // padded for 16-byte LDS reads
+    static constexpr int lds_doubles(int eb) {
+        const int nt = eb * NP;
+        const int a = (9 + (PML ? 24 : 0)) * nt;   // q and PML flux columns
+        const int b = eb * NFP * NFCP;             // face values
+        const int c = nt * NV;                     // volume values (aliases a and b)
+        const int e = 16 * nt;                     // energy (aliases everything)
+        const int ab = a + b;
+        const int m = ab > c ? ab : c;
+        return m > e ? m : e;
+    }
+    static constexpr int pick_eb() {
+        int eb = (256 / NP) > 0 ? (256 / NP) : 1;
+        while (eb > 1 && lds_doubles(eb) * 8 > 76 * 1024) --eb;
+        return eb;
+    }
+    static constexpr int EB = pick_eb();
+    static constexpr int NT = EB * NP;
+    static constexpr int NCT = EB * NCOL;
+    static constexpr int NTHR = ((NT > NCT ? NT : NCT) + 63) / 64 * 64 > 256 ? 256
+                                : ((NT > NCT ? NT : NCT) + 63) / 64 * 64;
+    static constexpr int LDS = lds_doubles(EB);
+};
+
+template <typename R>
+using cptr = const __attribute__((address_space(4))) R *;   // uniform, read-only: scalar loads
+
 template <int ORDER, bool ATT, bool PML, typename R>
-__global__ void __launch_bounds__(Dims<ORDER>::NT)
+__global__ void __launch_bounds__((Shape<ORDER, ATT, PML>::NTHR))
 stage_kernel(StageArgs<R> a) {
-    using Dm = Dims<ORDER>;
-    constexpr int NP = Dm::NP, NPF = Dm::NPF, NFP = Dm::NFP, EB = Dm::EB, NT = Dm::NT;
-    constexpr int NFC = ATT ? 15 : 9;                    // face values per face node
-    constexpr int XSZ = PML ? kPmlCols * NT : 0;
-    constexpr int FSZ = EB * NFP * NFC;
-    constexpr int USZ = (XSZ > FSZ ? XSZ : FSZ) > 15 * NT ? (XSZ > FSZ ? XSZ : FSZ) : 15 * NT;
-    __shared__ R sQ[9 * NT];
-    __shared__ R sU[USZ];                                // PML flux columns / face values / energy
-    __shared__ R sRed[NT];
+    using S = Shape<ORDER, ATT, PML>;
+    constexpr int NP = S::NP, NPF = S::NPF, NFP = S::NFP, EB = S::EB, NT = S::NT;
+    constexpr int NCOL = S::NCOL, NV = S::NV, NFC = S::NFC, NFCP = S::NFCP, NTHR = S::NTHR;
+    __shared__ __align__(16) R lds[S::LDS];
+    R *sQ = lds;                          // [9][NT]
+    R *sX = lds + 9 * NT;                 // [24][NT]    PML flux columns
+    R *sF = lds + (9 + (PML ? 24 : 0)) * NT;   // [EB][NFP][NFCP]
+    R *sV = lds;                          // [NT][NV]    after the columns are in registers
+    constexpr int NRED = NTHR <= 64 ? 64 : (NTHR <= 128 ? 128 : 256);   // power of two >= NTHR
+    __shared__ R sRed[NRED];
 
     const int t = threadIdx.x;
-    const int eb = t / NP, j = t - eb * NP;
-    const int idx = blockIdx.x * EB + eb;
-    const bool valid = idx < a.nlist;
-    const int e = valid ? a.elist[idx] : 0;
     const size_t ng = (size_t)a.nglob;
+    const int it_step = *a.d_it;
+
+    // ---- node-thread identity ------------------------------------------------
+    const bool node_thr = t < NT;
+    const int eb = node_thr ? t / NP : 0, j = node_thr ? t - (t / NP) * NP : 0;
+    const int idx = blockIdx.x * EB + eb;
+    const bool valid = node_thr && idx < a.nlist;
+    const int e = valid ? a.elist[idx] : 0;
     const size_t g = (size_t)e * NP + j;
 
     R qs[9];
+    R ef0 = 0, ef1 = 0, ef2 = 0, ef3 = 0, rho = 1;
+    if (node_thr) {
 #pragma unroll
-    for (int f = 0; f < 9; ++f) {
-        qs[f] = valid ? a.qin[f * ng + g] : R(0);
-        sQ[f * NT + t] = qs[f];
-    }
-    R ef0 = 0, ef1 = 0, ef2 = 0, ef3 = 0, vp = 0, vs = 0, rho = 1, lam = 0, mu = 0;
-    if (valid) {
-        const R *mc = a.matc + (size_t)e * 8;
-        vp = mc[0]; vs = mc[1]; rho = mc[2]; lam = mc[3]; mu = mc[4];
-        ef0 = -(lam + 2 * mu); ef1 = -lam; ef2 = -mu; ef3 = R(-1.0) / rho;
-    }
-    // PML: modified fluxes (src/timeloopMod.f90:399-405)
-    R Fm[9], Gm[9], Hm[9], pcur[27];
-    if constexpr (PML) {
-        elastic_flux_node(qs, ef0, ef1, ef2, ef3, Fm, Gm, Hm);
-        R kx = 1, ky = 1, kz = 1;
+        for (int f = 0; f < 9; ++f) {
+            qs[f] = valid ? a.qin[f * ng + g] : R(0);
+            sQ[f * NT + t] = qs[f];
+        }
         if (valid) {
-            kx = a.pmlc[6 * ng + g]; ky = a.pmlc[7 * ng + g]; kz = a.pmlc[8 * ng + g];
+            const R *mc = a.matc + (size_t)e * 8;
+            rho = mc[2];
+            ef0 = -(mc[3] + 2 * mc[4]); ef1 = -mc[3]; ef2 = -mc[4]; ef3 = R(-1.0) / rho;
         }
+    }
+    if constexpr (PML) {
+        if (node_thr) {
+            // modified fluxes (src/timeloopMod.f90:399-405) and the ADE update
+            // of f',g',h' (src/rungekuttaMod.f90:52-63); the update needs only
+            // stage-start values, so it is done here and the auxiliaries die
+            R Fm[9], Gm[9], Hm[9];
+            elastic_flux_node(qs, ef0, ef1, ef2, ef3, Fm, Gm, Hm);
+            R kk[3] = {1, 1, 1}, al[3] = {0, 0, 0}, dk[3] = {0, 0, 0};
+            if (valid) {
 #pragma unroll
-        for (int i = 0; i < 27; ++i) pcur[i] = valid ? a.pin[i * ng + g] : R(0);
+                for (int d = 0; d < 3; ++d) {
+                    al[d] = a.pmlc[d * ng + g]; dk[d] = a.pmlc[(3 + d) * ng + g]; kk[d] = a.pmlc[(6 + d) * ng + g];
+                }
+            }
+            int col = 0;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            Fm[i] = (pcur[i] + Fm[i]) / kx;
-            Gm[i] = (pcur[9 + i] + Gm[i]) / ky;
-            Hm[i] = (pcur[18 + i] + Hm[i]) / kz;
+            for (int d = 0; d < 3; ++d) {
+                R *FF = d == 0 ? Fm : (d == 1 ? Gm : Hm);
+                const int skip = d == 0 ? 4 : (d == 1 ? 5 : 3);
+#pragma unroll
+                for (int i = 0; i < 9; ++i) {
+                    const int c = d * 9 + i;
+                    const R cur = valid ? a.pin[c * ng + g] : R(0);
+                    const R mod = (cur + FF[i]) / kk[d];
+                    if (i != skip) sX[(col++) * NT + t] = mod;
+                    const R use = a.pml_compat ? a.last_flux[c * NP + j] : mod;
+                    const R dadt = -al[d] * cur - dk[d] * (cur + use);
+                    const R base = (a.irk == 1 || !valid) ? cur : a.pbase[c * ng + g];
+                    const R v = rk_combine<R>(a.scheme, a.irk, cur, base, dadt, a.dt, a.w13, a.w23);
+                    if (valid) a.pout[c * ng + g] = v;
+                }
+            }
         }
-        int c = 0;
-#pragma unroll
-        for (int i = 0; i < 9; ++i) if (i != 4) sU[(c++) * NT + t] = Fm[i];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) if (i != 5) sU[(c++) * NT + t] = Gm[i];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) if (i != 3) sU[(c++) * NT + t] = Hm[i];
     }
     __syncthreads();
 
-    // ---- volume term ------------------------------------------------------
-    R met[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) met[k] = valid ? a.metric[k * ng + g] : R(0);
-    const R rx = met[0], ry = met[1], rz = met[2], sx = met[3], sy = met[4], sz = met[5];
-    const R tx = met[6], ty = met[7], tz = met[8];
-    const R *Dr = a.D + j * NP, *Ds = a.D + NP * NP + j * NP, *Dt = a.D + 2 * NP * NP + j * NP;
-    const int ebase = eb * NP;
-    R rq[9];
-    R gv[9];   // velocity gradients d(v_c)/d(x_k), c-major: gv[c*3+k]
-    if constexpr (!PML) {
-        R dr[9], ds[9], dt[9];
-#pragma unroll
-        for (int f = 0; f < 9; ++f) { dr[f] = 0; ds[f] = 0; dt[f] = 0; }
-        for (int m = 0; m < NP; ++m) {
-            const R cr = Dr[m], cs = Ds[m], ct = Dt[m];
-#pragma unroll
-            for (int f = 0; f < 9; ++f) {
-                const R v = sQ[f * NT + ebase + m];
-                dr[f] += cr * v; ds[f] += cs * v; dt[f] += ct * v;
-            }
-        }
-        R dx[9], dy[9], dz[9];
-#pragma unroll
-        for (int f = 0; f < 9; ++f) {
-            dx[f] = rx * dr[f] + sx * ds[f] + tx * dt[f];
-            dy[f] = ry * dr[f] + sy * ds[f] + ty * dt[f];
-            dz[f] = rz * dr[f] + sz * ds[f] + tz * dt[f];
-        }
-        rq[0] = ef0 * dx[6] + ef1 * dy[7] + ef1 * dz[8];
-        rq[1] = ef1 * dx[6] + ef0 * dy[7] + ef1 * dz[8];
-        rq[2] = ef1 * dx[6] + ef1 * dy[7] + ef0 * dz[8];
-        rq[3] = ef2 * (dx[7] + dy[6]);
-        rq[4] = ef2 * (dy[8] + dz[7]);
-        rq[5] = ef2 * (dx[8] + dz[6]);
-        rq[6] = ef3 * (dx[0] + dy[3] + dz[5]);
-        rq[7] = ef3 * (dx[3] + dy[1] + dz[4]);
-        rq[8] = ef3 * (dx[5] + dy[4] + dz[2]);
-        if constexpr (ATT) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) { gv[c * 3 + 0] = dx[6 + c]; gv[c * 3 + 1] = dy[6 + c]; gv[c * 3 + 2] = dz[6 + c]; }
-        }
-    } else {
-        // 24 flux columns through Dr, Ds, Dt (src/timeloopMod.f90:407-420)
-        R vF[9], vG[9], vH[9];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) { vF[i] = 0; vG[i] = 0; vH[i] = 0; }
-#pragma unroll
-        for (int c = 0; c < kPmlCols; ++c) {
-            R sr = 0, ss = 0, st = 0;
-            const R *col = sU + c * NT + ebase;
-            for (int m = 0; m < NP; ++m) {
-                const R v = col[m];
-                sr += Dr[m] * v; ss += Ds[m] * v; st += Dt[m] * v;
-            }
-            if (c < 8) {
-                const int i = c < 4 ? c : c + 1;
-                vF[i] = rx * sr + sx * ss + tx * st;
-            } else if (c < 16) {
-                const int i = (c - 8) < 5 ? (c - 8) : (c - 8) + 1;
-                vG[i] = ry * sr + sy * ss + ty * st;
-            } else {
-                const int i = (c - 16) < 3 ? (c - 16) : (c - 16) + 1;
-                vH[i] = rz * sr + sz * ss + tz * st;
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < 9; ++i) rq[i] = vF[i] + vG[i] + vH[i];
-        if constexpr (ATT) {
-            R dr3[3] = {0, 0, 0}, ds3[3] = {0, 0, 0}, dt3[3] = {0, 0, 0};
-            for (int m = 0; m < NP; ++m) {
-                const R cr = Dr[m], cs = Ds[m], ct = Dt[m];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const R v = sQ[(6 + c) * NT + ebase + m];
-                    dr3[c] += cr * v; ds3[c] += cs * v; dt3[c] += ct * v;
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                gv[c * 3 + 0] = rx * dr3[c] + sx * ds3[c] + tx * dt3[c];
-                gv[c * 3 + 1] = ry * dr3[c] + sy * ds3[c] + ty * dt3[c];
-                gv[c * 3 + 2] = rz * dr3[c] + sz * ds3[c] + tz * dt3[c];
-            }
-        }
-    }
-    __syncthreads();   // sU is reused for face values below
-
-    // ---- face term: upwind flux per face node into sU ----------------------
-    for (int it = t; it < EB * NFP; it += NT) {
+    // ---- face term: upwind flux per face node into sF ----------------------
+    for (int it = t; it < EB * NFP; it += NTHR) {
         const int fb = it / NFP, k = it - fb * NFP;
         const int fidx = blockIdx.x * EB + fb;
-        if (fidx >= a.nlist) continue;
+        R *dst = sF + (fb * NFP + k) * NFCP;
+        if (fidx >= a.nlist) {
+#pragma unroll
+            for (int f = 0; f < NFCP; ++f) dst[f] = 0;
+            continue;
+        }
         const int fe = a.elist[fidx];
         const int fi = k / NPF, i = k - fi * NPF;
         const int own = fb * NP + a.fmask[fi * NPF + i];
@@ -341,116 +313,170 @@ stage_kernel(StageArgs<R> a) {
         upwind_flux<R>(a.frame + ((size_t)fe * 4 + fi) * 9, mc[0], mc[1], mc[2], mc[3], mc[4], ft == -1, dq, fl, af, ATT);
         const R sgn = ft == -2 ? R(-1) : R(1);
         const R fs = a.fscale[(size_t)fe * NFP + k];
-        R *dst = sU + (fb * NFP + k) * NFC;
 #pragma unroll
         for (int f = 0; f < 9; ++f) dst[f] = fs * (sgn * fl[f]) / 2;
         if constexpr (ATT) {
 #pragma unroll
             for (int f = 0; f < 6; ++f) dst[9 + f] = fs * (sgn * af[f]) / 2;
         }
+#pragma unroll
+        for (int f = NFC; f < NFCP; ++f) dst[f] = 0;
     }
     __syncthreads();
 
-    // ---- LIFT --------------------------------------------------------------
+    // ---- LIFT (node threads) and column loads (column threads) -------------
     R lf[NFC];
 #pragma unroll
     for (int f = 0; f < NFC; ++f) lf[f] = 0;
-    {
-        const R *src = sU + eb * NFP * NFC;
+    if (node_thr) {
+        const R *src = sF + eb * NFP * NFCP;
         for (int k = 0; k < NFP; ++k) {
             const R l = a.LT[k * NP + j];
+            const R *row = src + k * NFCP;
 #pragma unroll
-            for (int f = 0; f < NFC; ++f) lf[f] += l * src[k * NFC + f];
+            for (int f = 0; f < NFC; ++f) lf[f] += l * row[f];
         }
     }
-
-    const int it_step = *a.d_it;
-    R rhs[9];
+    // column thread: one column of one element in registers
+    const int ct = t;
+    const bool col_thr = ct < EB * NCOL;
+    const int ceb = col_thr ? ct / NCOL : 0, cc = col_thr ? ct - (ct / NCOL) * NCOL : 0;
+    const int cidx = blockIdx.x * EB + ceb;
+    const bool cvalid = col_thr && cidx < a.nlist;
+    const int ce = cvalid ? a.elist[cidx] : 0;
+    R X[NP];
+    {
+        const R *colp;
+        if constexpr (PML) colp = cc < 24 ? sX + cc * NT : sQ + (6 + cc - 24) * NT;
+        else colp = sQ + cc * NT;
+        colp += ceb * NP;
 #pragma unroll
-    for (int f = 0; f < 9; ++f) rhs[f] = rq[f];
-
-    // ---- attenuation (src/timeloopMod.f90:426-461) --------------------------
-    R thm[ATT ? 18 : 1], arq[ATT ? 18 : 1];
-    if constexpr (ATT) {
-#pragma unroll
-        for (int c = 0; c < 18; ++c) thm[c] = valid ? a.thin[c * ng + g] : R(0);
-        // strain-rate components of the anelastic flux divergence
-        const R ex = gv[0], ey = gv[4], ez = gv[8];
-        const R exy = gv[3] + gv[1];   // d vy/dx + d vx/dy
-        const R eyz = gv[7] + gv[5];   // d vz/dy + d vy/dz
-        const R exz = gv[6] + gv[2];   // d vz/dx + d vx/dz
-#pragma unroll
-        for (int i = 0; i < NMB; ++i) {
-            const R w = a.wl[i];
-            const R vol[6] = {-w * ex, -w * ey, -w * ez, R(-0.5) * w * exy, R(-0.5) * w * eyz, R(-0.5) * w * exz};
-#pragma unroll
-            for (int c = 0; c < 6; ++c)
-                arq[i * 6 + c] = vol[c] + w * lf[9 + c] - w * thm[i * 6 + c];
-        }
-        const R *av = a.anel + (size_t)e * NMB * 3;
-#pragma unroll
-        for (int i = 0; i < NMB; ++i) {
-            const R a1 = valid ? av[i * 3 + 0] : R(0), a2 = valid ? av[i * 3 + 1] : R(0), a3 = valid ? av[i * 3 + 2] : R(0);
-            const R *th = thm + i * 6;
-            rhs[0] = rhs[0] + a3 * th[0] + a1 * (th[1] + th[2]);
-            rhs[1] = rhs[1] + a3 * th[1] + a1 * (th[2] + th[0]);
-            rhs[2] = rhs[2] + a3 * th[2] + a1 * (th[0] + th[1]);
-            rhs[3] = rhs[3] + a2 * th[3];
-            rhs[4] = rhs[4] + a2 * th[4];
-            rhs[5] = rhs[5] + a2 * th[5];
-        }
+        for (int m = 0; m < NP; ++m) X[m] = col_thr ? colp[m] : R(0);
     }
-#pragma unroll
-    for (int f = 0; f < 9; ++f) rhs[f] = rhs[f] + lf[f];
+    __syncthreads();   // sQ/sX/sF are dead from here; sV aliases them
 
-    // ---- sources (src/timeloopMod.f90:467-475) -----------------------------
-    if (valid) {
-        for (int s = a.src_head[e]; s >= 0; s = a.src_next[s]) {
-            if (a.srctype[s] == 0) {
-                const R *sp = a.src_array + (((size_t)it_step * a.nsrc + s) * 3) * NP + j;
-                rhs[6] += sp[0]; rhs[7] += sp[NP]; rhs[8] += sp[2 * NP];
+    // ---- volume term: Dr/Ds/Dt applied to the column, metric per node ------
+    if (col_thr) {
+        cptr<R> Dc = (cptr<R>)a.D;
+        // which metric row(s) the column needs: PML flux columns one direction,
+        // q / velocity columns all three (src/timeloopMod.f90:417-419)
+        const int kind = PML ? (cc < 8 ? 0 : (cc < 16 ? 1 : (cc < 24 ? 2 : 3))) : 3;
+        const R *met = a.metric + (size_t)ce * NP;
+        for (int jr = 0; jr < NP; ++jr) {
+            R dr = 0, ds = 0, dt = 0;
+#pragma unroll
+            for (int m = 0; m < NP; ++m) {
+                const R x = X[m];
+                dr += Dc[jr * NP + m] * x;
+                ds += Dc[NP * NP + jr * NP + m] * x;
+                dt += Dc[2 * NP * NP + jr * NP + m] * x;
+            }
+            R *out = sV + (ceb * NP + jr) * NV;
+            if (kind < 3) {
+                const R m0 = met[(kind + 0) * ng + jr], m1 = met[(kind + 3) * ng + jr], m2 = met[(kind + 6) * ng + jr];
+                out[cc] = m0 * dr + m1 * ds + m2 * dt;
             } else {
-                const R *sp = a.src_array_m + (((size_t)it_step * a.nsrc + s) * 6) * NP + j;
+                const R rx = met[0 * ng + jr], ry = met[1 * ng + jr], rz = met[2 * ng + jr];
+                const R sx = met[3 * ng + jr], sy = met[4 * ng + jr], sz = met[5 * ng + jr];
+                const R tx = met[6 * ng + jr], ty = met[7 * ng + jr], tz = met[8 * ng + jr];
+                const int base = PML ? 24 + (cc - 24) * 3 : cc * 3;
+                out[base + 0] = rx * dr + sx * ds + tx * dt;
+                out[base + 1] = ry * dr + sy * ds + ty * dt;
+                out[base + 2] = rz * dr + sz * ds + tz * dt;
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- assemble the right-hand side (node threads) -----------------------
+    R rhs[9];
+    R gv[9];   // velocity gradients: gv[c*3+k] = d v_c / d x_k
+    if (node_thr) {
+        const R *v = sV + t * NV;
+        if constexpr (!PML) {
+            R dx[9], dy[9], dz[9];
 #pragma unroll
-                for (int c = 0; c < 6; ++c) rhs[c] += sp[c * NP];
+            for (int f = 0; f < 9; ++f) { dx[f] = v[f * 3]; dy[f] = v[f * 3 + 1]; dz[f] = v[f * 3 + 2]; }
+            rhs[0] = ef0 * dx[6] + ef1 * dy[7] + ef1 * dz[8];
+            rhs[1] = ef1 * dx[6] + ef0 * dy[7] + ef1 * dz[8];
+            rhs[2] = ef1 * dx[6] + ef1 * dy[7] + ef0 * dz[8];
+            rhs[3] = ef2 * (dx[7] + dy[6]);
+            rhs[4] = ef2 * (dy[8] + dz[7]);
+            rhs[5] = ef2 * (dx[8] + dz[6]);
+            rhs[6] = ef3 * (dx[0] + dy[3] + dz[5]);
+            rhs[7] = ef3 * (dx[3] + dy[1] + dz[4]);
+            rhs[8] = ef3 * (dx[5] + dy[4] + dz[2]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) { gv[c * 3] = dx[6 + c]; gv[c * 3 + 1] = dy[6 + c]; gv[c * 3 + 2] = dz[6 + c]; }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                const R vF = i == 4 ? R(0) : v[i < 4 ? i : i - 1];
+                const R vG = i == 5 ? R(0) : v[8 + (i < 5 ? i : i - 1)];
+                const R vH = i == 3 ? R(0) : v[16 + (i < 3 ? i : i - 1)];
+                rhs[i] = vF + vG + vH;
+            }
+            if constexpr (ATT) {
+#pragma unroll
+                for (int c = 0; c < 9; ++c) gv[c] = v[24 + c];
             }
         }
     }
 
-    // ---- Runge-Kutta update -----------------------------------------------
-    R qnew[9];
-#pragma unroll
-    for (int f = 0; f < 9; ++f) {
-        const R base = (a.irk == 1 || !valid) ? qs[f] : a.qbase[f * ng + g];
-        qnew[f] = rk_combine<R>(a.scheme, a.irk, qs[f], base, rhs[f], a.dt, a.w13, a.w23);
-        if (valid) a.qout[f * ng + g] = qnew[f];
-    }
+    // ---- attenuation (src/timeloopMod.f90:426-461), one mechanism at a time --
     if constexpr (ATT) {
+        if (node_thr) {
+            const R ex = gv[0], ey = gv[4], ez = gv[8];
+            const R exy = gv[3] + gv[1], eyz = gv[7] + gv[5], exz = gv[6] + gv[2];
+            const R *av = a.anel + (size_t)e * NMB * 3;
 #pragma unroll
-        for (int c = 0; c < 18; ++c) {
-            const R base = (a.irk == 1 || !valid) ? thm[c] : a.thbase[c * ng + g];
-            const R v = rk_combine<R>(a.scheme, a.irk, thm[c], base, arq[c], a.dt, a.w13, a.w23);
-            if (valid) a.thout[c * ng + g] = v;
+            for (int i = 0; i < NMB; ++i) {
+                const R w = a.wl[i];
+                const R vol[6] = {-w * ex, -w * ey, -w * ez, R(-0.5) * w * exy, R(-0.5) * w * eyz, R(-0.5) * w * exz};
+                R th[6];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) th[c] = valid ? a.thin[(i * 6 + c) * ng + g] : R(0);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    const R arq = vol[c] + w * lf[9 + c] - w * th[c];
+                    const R base = (a.irk == 1 || !valid) ? th[c] : a.thbase[(i * 6 + c) * ng + g];
+                    const R nv = rk_combine<R>(a.scheme, a.irk, th[c], base, arq, a.dt, a.w13, a.w23);
+                    if (valid) a.thout[(i * 6 + c) * ng + g] = nv;
+                }
+                const R a1 = valid ? av[i * 3 + 0] : R(0), a2 = valid ? av[i * 3 + 1] : R(0), a3 = valid ? av[i * 3 + 2] : R(0);
+                rhs[0] = rhs[0] + a3 * th[0] + a1 * (th[1] + th[2]);
+                rhs[1] = rhs[1] + a3 * th[1] + a1 * (th[2] + th[0]);
+                rhs[2] = rhs[2] + a3 * th[2] + a1 * (th[0] + th[1]);
+                rhs[3] = rhs[3] + a2 * th[3];
+                rhs[4] = rhs[4] + a2 * th[4];
+                rhs[5] = rhs[5] + a2 * th[5];
+            }
         }
     }
-    if constexpr (PML) {
-        // ADE update of the auxiliary fluxes (src/rungekuttaMod.f90:52-63)
-        R al[3] = {1, 1, 1}, dk[3] = {0, 0, 0};
+
+    R qnew[9];
+    if (node_thr) {
+#pragma unroll
+        for (int f = 0; f < 9; ++f) rhs[f] = rhs[f] + lf[f];
+        // sources (src/timeloopMod.f90:467-475)
         if (valid) {
+            for (int s = a.src_head[e]; s >= 0; s = a.src_next[s]) {
+                if (a.srctype[s] == 0) {
+                    const R *sp = a.src_array + (((size_t)it_step * a.nsrc + s) * 3) * NP + j;
+                    rhs[6] += sp[0]; rhs[7] += sp[NP]; rhs[8] += sp[2 * NP];
+                } else {
+                    const R *sp = a.src_array_m + (((size_t)it_step * a.nsrc + s) * 6) * NP + j;
 #pragma unroll
-            for (int d = 0; d < 3; ++d) { al[d] = a.pmlc[d * ng + g]; dk[d] = a.pmlc[(3 + d) * ng + g]; }
+                    for (int c = 0; c < 6; ++c) rhs[c] += sp[c * NP];
+                }
+            }
         }
+        // Runge-Kutta update
 #pragma unroll
-        for (int c = 0; c < 27; ++c) {
-            const int d = c / 9, i = c - d * 9;
-            R use;
-            if (a.pml_compat) use = a.last_flux[c * NP + j];
-            else use = d == 0 ? Fm[i] : (d == 1 ? Gm[i] : Hm[i]);
-            const R cur = pcur[c];
-            const R dadt = -al[d] * cur - dk[d] * (cur + use);
-            const R base = (a.irk == 1 || !valid) ? cur : a.pbase[c * ng + g];
-            const R v = rk_combine<R>(a.scheme, a.irk, cur, base, dadt, a.dt, a.w13, a.w23);
-            if (valid) a.pout[c * ng + g] = v;
+        for (int f = 0; f < 9; ++f) {
+            const R base = (a.irk == 1 || !valid) ? qs[f] : a.qbase[f * ng + g];
+            qnew[f] = rk_combine<R>(a.scheme, a.irk, qs[f], base, rhs[f], a.dt, a.w13, a.w23);
+            if (valid) a.qout[f * ng + g] = qnew[f];
         }
     }
 
@@ -472,35 +498,38 @@ stage_kernel(StageArgs<R> a) {
             }
         }
         if (a.blk_energy) {
-            // energy partial of this block: 0.5*J*(rho v.Mv + sigma.M e) per element
+            // energy partial of this block (src/timeloopMod.f90:490-516)
             __syncthreads();
+            R *sE = lds;   // [15][NT]
             R lam_e = 0, mu_e = 0;
             if (valid) { mu_e = a.emat[(size_t)e * 2]; lam_e = a.emat[(size_t)e * 2 + 1]; }
-            const R temp1 = R(1.) / (4 * mu_e * mu_e + 6 * mu_e * lam_e);
-            const R temp2 = 2 * mu_e;
-            R y[15];
+            if (node_thr) {
+                const R temp1 = R(1.) / (4 * mu_e * mu_e + 6 * mu_e * lam_e);
+                const R temp2 = 2 * mu_e;
+                R y[15];
 #pragma unroll
-            for (int c = 0; c < 9; ++c) y[c] = qnew[c];
-            y[9] = (2 * (lam_e + mu_e) * qnew[0] - lam_e * (qnew[1] + qnew[2])) / temp1;
-            y[10] = (2 * (lam_e + mu_e) * qnew[1] - lam_e * (qnew[0] + qnew[2])) / temp1;
-            y[11] = (2 * (lam_e + mu_e) * qnew[2] - lam_e * (qnew[0] + qnew[0])) / temp1;
-            y[12] = qnew[3] / temp2; y[13] = qnew[4] / temp2; y[14] = qnew[5] / temp2;
+                for (int c = 0; c < 9; ++c) y[c] = qnew[c];
+                y[9] = (2 * (lam_e + mu_e) * qnew[0] - lam_e * (qnew[1] + qnew[2])) / temp1;
+                y[10] = (2 * (lam_e + mu_e) * qnew[1] - lam_e * (qnew[0] + qnew[2])) / temp1;
+                y[11] = (2 * (lam_e + mu_e) * qnew[2] - lam_e * (qnew[0] + qnew[0])) / temp1;
+                y[12] = qnew[3] / temp2; y[13] = qnew[4] / temp2; y[14] = qnew[5] / temp2;
 #pragma unroll
-            for (int c = 0; c < 15; ++c) sU[c * NT + t] = valid ? y[c] : R(0);
-            __syncthreads();
-            // node j's share: x_j * (M y)_j for the 9 quadratic forms
-            R mv[9];
-#pragma unroll
-            for (int c = 0; c < 9; ++c) mv[c] = 0;
-            for (int k = 0; k < NP; ++k) {
-                const R mjk = a.mass[j * NP + k];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) mv[6 + c] += mjk * sU[(6 + c) * NT + ebase + k];
-#pragma unroll
-                for (int c = 0; c < 6; ++c) mv[c] += mjk * sU[(9 + c) * NT + ebase + k];
+                for (int c = 0; c < 15; ++c) sE[c * NT + t] = valid ? y[c] : R(0);
             }
+            __syncthreads();
             R part = 0;
             if (valid) {
+                R mv[9];
+#pragma unroll
+                for (int c = 0; c < 9; ++c) mv[c] = 0;
+                const int ebase = eb * NP;
+                for (int k = 0; k < NP; ++k) {
+                    const R mjk = a.mass[j * NP + k];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) mv[6 + c] += mjk * sE[(6 + c) * NT + ebase + k];
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) mv[c] += mjk * sE[(9 + c) * NT + ebase + k];
+                }
                 const R jac0 = a.jac[(size_t)e * NP];
                 const R d = rho * (qnew[6] * mv[6]) + rho * (qnew[7] * mv[7]) + rho * (qnew[8] * mv[8]) +
                             qnew[0] * mv[0] + qnew[1] * mv[1] + qnew[2] * mv[2] +
@@ -508,12 +537,13 @@ stage_kernel(StageArgs<R> a) {
                 part = R(0.5) * jac0 * d;
             }
             sRed[t] = part;
+            if (t + NTHR < NRED) sRed[t + NTHR] = 0;
             __syncthreads();
-            if (t == 0) {
-                R acc = 0;
-                for (int k = 0; k < NT; ++k) acc += sRed[k];
-                a.blk_energy[blockIdx.x] = acc;
+            for (int s = NRED / 2; s > 0; s >>= 1) {
+                if (t < s) sRed[t] += sRed[t + s];
+                __syncthreads();
             }
+            if (t == 0) a.blk_energy[blockIdx.x] = sRed[0];
         }
     }
 }
@@ -551,14 +581,17 @@ __global__ void last_flux_kernel(const R *qin, const R *pin, const R *pmlc, cons
     }
 }
 
-// Receiver sampling (src/timeloopMod.f90:358-373) of row `row0 + *d_it`,
-// energy reduction of the step, and the step-counter advance.  One block.
+// Receiver sampling (src/timeloopMod.f90:358-373) of row `row_offset + *d_it`,
+// the energy of the step (sum of the stage kernels' block partials in a fixed
+// order), and the step-counter advance.  One block of 1024 threads.
 template <int ORDER, typename R>
-__global__ void step_end_kernel(const R *q, const R *u_slot, const R *a_slot, const R *rec_int, const int *recelem,
-                                const int *rec_slot_of, int nrec, int nglob, int nt, int row_offset, int advance,
-                                R *seis, const R *blk_energy, int nblk, R *energy_out, int *d_it) {
+__global__ void __launch_bounds__(1024)
+step_end_kernel(const R *q, const R *u_slot, const R *a_slot, const R *rec_int, const int *recelem,
+                const int *rec_slot_of, int nrec, int nglob, int nt, int row_offset, int advance,
+                R *seis, const R *blk_energy, int nblk, R *energy_out, int *d_it) {
     constexpr int NP = Dims<ORDER>::NP;
     __shared__ int s_it;
+    __shared__ R s_red[1024];
     if (threadIdx.x == 0) s_it = *d_it;
     __syncthreads();
     const int it = s_it;
@@ -579,10 +612,16 @@ __global__ void step_end_kernel(const R *q, const R *u_slot, const R *a_slot, co
             seis[((size_t)k * nt + row) * nrec + r] = acc;
         }
     }
-    if (advance && blk_energy && threadIdx.x == 0) {
+    if (advance && blk_energy) {
         R acc = 0;
-        for (int b = 0; b < nblk; ++b) acc += blk_energy[b];
-        energy_out[it] = acc;
+        for (int b = threadIdx.x; b < nblk; b += blockDim.x) acc += blk_energy[b];
+        s_red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) s_red[threadIdx.x] += s_red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) energy_out[it] = s_red[0];
     }
     __syncthreads();
     if (advance && threadIdx.x == 0) *d_it = it + 1;
