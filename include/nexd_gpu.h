@@ -158,6 +158,67 @@ void nexd_destroy(nexd_ctx *ctx);
  * failure when ctx is NULL. */
 const char *nexd_last_error(const nexd_ctx *ctx);
 
+/* ==========================================================================
+ * Batched shots (no reference counterpart: the reference runs one source
+ * configuration per job).  Many shots on one mesh advance together in every
+ * stage launch; geometry, operators and receivers are shared, materials,
+ * PML profiles and sources are per shot.  Arithmetic is FP32 (the fast path)
+ * or FP64.  Sources are given factored: a spatial operand per source and
+ * field times the source time function of the step (the reference's
+ * srcArray(:,c,i,it) = spatial(:,c,i)*stf(it,i), src/timeloopMod.f90:327-356).
+ * ========================================================================== */
+#define NEXD_FP64 0
+#define NEXD_FP32 1
+
+typedef struct nexd_batch_config {
+    int nelem, order, nshot;
+    int nsrc;             /* sources per shot */
+    int nrec;             /* receivers, shared by all shots */
+    int nt, timeint, attenuation, pml, compat;
+    int precision;        /* NEXD_FP32 or NEXD_FP64 */
+    int shared_material;  /* 1: one material/PML set for every shot */
+    int device;
+    double dt;
+} nexd_batch_config;
+
+typedef struct nexd_batch nexd_batch;
+
+int nexd_batch_create(const nexd_batch_config *cfg, nexd_batch **out);
+/* same arguments and layouts as nexd_upload_operators / nexd_upload_mesh */
+int nexd_batch_upload_operators(nexd_batch *b, const double *dr, const double *ds, const double *dt,
+                                const double *lift, const double *mass, const int *fmask);
+int nexd_batch_upload_mesh(nexd_batch *b, const int *ibool,
+                           const double *rx, const double *ry, const double *rz,
+                           const double *sx, const double *sy, const double *sz,
+                           const double *tx, const double *ty, const double *tz,
+                           const double *jacobian, const double *fscale,
+                           const int *neighbor, const int *face, const int *ibt, const int *ibn,
+                           const double *nnormal, const double *stangential, const double *ttangential);
+/* per shot (shot 0..nshot-1; with shared_material only shot 0) */
+int nexd_batch_upload_material(nexd_batch *b, int shot, const double *vpu, const double *vsu,
+                               const double *rho, const double *lambdau, const double *muu);
+int nexd_batch_upload_anelastic(nexd_batch *b, int shot, const double *wl, const double *anelasticvar);
+/* pmlcheck is a mesh property and must be the same for every shot */
+int nexd_batch_upload_pml(nexd_batch *b, int shot, const int *pmlcheck,
+                          const double *ddx, const double *ddy, const double *ddz,
+                          const double *alphax, const double *alphay, const double *alphaz,
+                          const double *kx, const double *ky, const double *kz);
+/* srcelem (nsrc, 1-based), srctype (nsrc), spatial (Np,6,nsrc): columns 1..3
+ * of a force source, 1..6 of a moment-tensor source; stf (nt,nsrc) */
+int nexd_batch_upload_sources(nexd_batch *b, int shot, const int *srcelem, const int *srctype,
+                              const double *spatial, const double *stf);
+int nexd_batch_upload_receivers(nexd_batch *b, const int *recelem, const double *rec_int);
+int nexd_batch_reset(nexd_batch *b);
+int nexd_batch_run(nexd_batch *b, int it_begin, int it_end);
+/* seismograms of one shot, each (nrec,nt) in double */
+int nexd_batch_get_seismograms(nexd_batch *b, int shot, double *ux, double *uy, double *uz,
+                               double *vx, double *vy, double *vz, double *ax, double *ay, double *az);
+int nexd_batch_last_run_ms(nexd_batch *b, double *ms);
+/* measurement: mean duration of one RK-stage-1 launch of element list 0/1 */
+int nexd_batch_time_stage_kernel(nexd_batch *b, int list, int nrep, double *avg_ms, int *nelem_launch);
+void nexd_batch_destroy(nexd_batch *b);
+const char *nexd_batch_last_error(const nexd_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

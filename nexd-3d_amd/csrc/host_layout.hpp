@@ -1,0 +1,120 @@
+// Host-side conversion of the reference's arrays (Fortran column-major,
+// 1-based) into the device layout of the stage kernels, with the checks the
+// C-ABI promises.  Shared by the single-shot and the batched contexts.
+#pragma once
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace nexd {
+
+struct OpsHost {
+    std::vector<double> D;    // [3][NP][NP]
+    std::vector<double> LT;   // [NFP][NP]
+    std::vector<double> M;    // [NP][NP]
+    std::vector<int> fm;      // [4][NPF] 0-based
+};
+
+inline std::string convert_operators(int np, int npf, const double *dr, const double *ds, const double *dt,
+                                     const double *lift, const double *mass, const int *fmask, OpsHost &o) {
+    if (!dr || !ds || !dt || !lift || !mass || !fmask) return "null operator";
+    const int nfp = 4 * npf;
+    o.D.assign((size_t)3 * np * np, 0.0);
+    o.LT.assign((size_t)nfp * np, 0.0);
+    o.M.assign((size_t)np * np, 0.0);
+    o.fm.assign((size_t)4 * npf, 0);
+    const double *src[3] = {dr, ds, dt};
+    for (int d = 0; d < 3; ++d)
+        for (int i = 0; i < np; ++i)
+            for (int m = 0; m < np; ++m) o.D[((size_t)d * np + i) * np + m] = src[d][(size_t)m * np + i];
+    for (int k = 0; k < nfp; ++k)
+        for (int j = 0; j < np; ++j) o.LT[(size_t)k * np + j] = lift[(size_t)k * np + j];
+    for (int i = 0; i < np; ++i)
+        for (int k = 0; k < np; ++k) o.M[(size_t)i * np + k] = mass[(size_t)k * np + i];
+    for (int f = 0; f < 4; ++f)
+        for (int i = 0; i < npf; ++i) {
+            const int v = fmask[f * npf + i] - 1;
+            if (v < 0 || v >= np) return "fmask entry out of range";
+            o.fm[f * npf + i] = v;
+        }
+    return "";
+}
+
+struct MeshHost {
+    std::vector<double> metric;  // [9][nglob]
+    std::vector<double> jac;     // [nglob]
+    std::vector<double> fscale;  // [nelem][NFP]
+    std::vector<double> frame;   // [nelem][4][9]
+    std::vector<int> nbr;        // [nelem][4][NPF]
+    std::vector<int> ftype;      // [nelem][4]
+};
+
+// err is "" on success; unsupported features are reported with a leading '!'
+inline std::string convert_mesh(int nelem, int np, int npf, const std::vector<int> &fm, const int *ibool,
+                                const double *const metric_in[9], const double *jacobian, const double *fscale,
+                                const int *neighbor, const int *face, const int *ibt, const int *ibn,
+                                const double *nnormal, const double *stangential, const double *ttangential,
+                                MeshHost &m) {
+    for (int k = 0; k < 9; ++k)
+        if (!metric_in[k]) return "null mesh array";
+    if (!ibool || !jacobian || !fscale || !neighbor || !face || !ibt || !ibn || !nnormal || !stangential ||
+        !ttangential)
+        return "null mesh array";
+    const size_t ng = (size_t)nelem * np;
+    const int nfp = 4 * npf;
+    for (size_t i = 0; i < ng; ++i)
+        if (ibool[i] != (int)i + 1) return "!ibool must be (ie-1)*Np+j";
+    m.metric.resize(9 * ng);
+    for (int k = 0; k < 9; ++k) std::memcpy(m.metric.data() + k * ng, metric_in[k], sizeof(double) * ng);
+    m.jac.assign(jacobian, jacobian + ng);
+    m.fscale.assign(fscale, fscale + (size_t)nelem * nfp);
+    m.frame.resize((size_t)nelem * 36);
+    m.nbr.resize((size_t)nelem * 4 * npf);
+    m.ftype.resize((size_t)nelem * 4);
+    for (int e = 0; e < nelem; ++e) {
+        for (int f = 0; f < 4; ++f) {
+            for (int k = 0; k < 3; ++k) {
+                m.frame[(e * 4 + f) * 9 + k] = nnormal[(e * 4 + f) * 3 + k];
+                m.frame[(e * 4 + f) * 9 + 3 + k] = stangential[(e * 4 + f) * 3 + k];
+                m.frame[(e * 4 + f) * 9 + 6 + k] = ttangential[(e * 4 + f) * 3 + k];
+            }
+            const int n = neighbor[e * 4 + f], fc = face[e * 4 + f];
+            for (int i = 0; i < npf; ++i) {
+                const size_t idx = ((size_t)e * 4 + f) * npf + i;
+                if ((size_t)ibt[idx] != (size_t)e * np + fm[f * npf + i] + 1)
+                    return "!ibt must be ibool(fmask(:,is),ie)";
+            }
+            if (n > 0) {
+                if (n > nelem) return "neighbor out of range";
+                if (fc < 1 || fc > 4) return "face of an interior face must be 1..4";
+                m.ftype[e * 4 + f] = fc - 1;
+                for (int i = 0; i < npf; ++i) {
+                    const size_t idx = ((size_t)e * 4 + f) * npf + i;
+                    const int v = ibn[idx] - 1;
+                    if (v < 0 || (size_t)v >= ng) return "ibn out of range";
+                    m.nbr[idx] = v;
+                }
+            } else if (n == 0) {
+                if (fc != -1 && fc != -2) return "boundary face must be free (-1) or absorbing (-2)";
+                m.ftype[e * 4 + f] = fc;
+                for (int i = 0; i < npf; ++i) m.nbr[((size_t)e * 4 + f) * npf + i] = -1;
+            } else {
+                return "!partition interfaces (neighbor == -1) are not supported";
+            }
+        }
+    }
+    return "";
+}
+
+inline void pml_coefficients(size_t ng, const double *ddx, const double *ddy, const double *ddz,
+                             const double *alphax, const double *alphay, const double *alphaz, const double *kx,
+                             const double *ky, const double *kz, std::vector<double> &pc) {
+    pc.resize(9 * ng);
+    for (size_t g = 0; g < ng; ++g) {
+        pc[0 * ng + g] = alphax[g]; pc[1 * ng + g] = alphay[g]; pc[2 * ng + g] = alphaz[g];
+        pc[3 * ng + g] = ddx[g] / kx[g]; pc[4 * ng + g] = ddy[g] / ky[g]; pc[5 * ng + g] = ddz[g] / kz[g];
+        pc[6 * ng + g] = kx[g]; pc[7 * ng + g] = ky[g]; pc[8 * ng + g] = kz[g];
+    }
+}
+
+}  // namespace nexd

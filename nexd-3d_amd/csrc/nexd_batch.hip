@@ -1,0 +1,719 @@
+// Batched shots: the nexd_batch_* part of include/nexd_gpu.h.
+//
+// nshot shots on one mesh share operators, geometry and receivers and
+// advance together: every stage launch covers all shots, with the shot index
+// varying fastest across blocks so that the blocks of one element group run
+// back to back and reuse its geometry from L2.  Arithmetic is FP32 or FP64
+// (the same kernels as the single-shot path, instantiated for the type).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/nexd_gpu.h"
+#include "host_layout.hpp"
+#include "plain_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_batch_create_error;
+
+struct BatchBase {
+    virtual ~BatchBase() = default;
+    nexd_batch_config cfg{};
+    std::string err;
+};
+
+template <typename R>
+std::vector<R> cvt(const std::vector<double> &v) {
+    return std::vector<R>(v.begin(), v.end());
+}
+
+template <typename R>
+struct Batch : BatchBase {
+    int np = 0, npf = 0, nfp = 0, nrk = 0, nslot = 0;
+    size_t ng = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<void *> allocs;
+    R *q[3] = {nullptr, nullptr, nullptr};
+    R *th[2] = {nullptr, nullptr};
+    R *paux[2] = {nullptr, nullptr};
+    R *D = nullptr, *LT = nullptr, *mass = nullptr, *metric = nullptr, *fscale = nullptr, *frame = nullptr;
+    R *jac = nullptr;
+    int *fmask = nullptr, *nbr = nullptr, *ftype = nullptr;
+    std::vector<int> fmask_host;
+    R *matc = nullptr, *anel = nullptr, *pmlc = nullptr;
+    double wl[nexd::NMB] = {0, 0, 0};
+    std::vector<int> pmlcheck;
+    int *elist_plain = nullptr, *elist_pml = nullptr;
+    int n_plain = 0, n_pml = 0;
+    int *src_head = nullptr, *src_next = nullptr, *srctype = nullptr;
+    R *src_spatial = nullptr, *src_stf = nullptr;
+    int *recelem = nullptr, *rec_slot = nullptr, *rec_slot_of = nullptr;
+    R *rec_int = nullptr, *u_slot = nullptr, *a_slot = nullptr, *seis = nullptr, *last_flux = nullptr;
+    int *d_it = nullptr;
+    int step = 0;
+    bool have_ops = false, have_mesh = false, initialized = false;
+    std::vector<char> have_mat, have_att, have_pml, have_src;
+    bool have_rec = false;
+    hipGraphExec_t graph = nullptr;
+    float last_ms = 0.f;
+
+    ~Batch() override {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (graph) (void)hipGraphExecDestroy(graph);
+        for (void *p : allocs) (void)hipFree(p);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    int fail(int code, const std::string &m) {
+        err = m;
+        return code;
+    }
+
+    template <typename T>
+    int dalloc(T **p, size_t n) {
+        void *v = nullptr;
+        if (n == 0) n = 1;
+        hipError_t e = hipMalloc(&v, n * sizeof(T));
+        if (e != hipSuccess) return fail(NEXD_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+        allocs.push_back(v);
+        *p = static_cast<T *>(v);
+        return 0;
+    }
+
+    template <typename T>
+    int put(T *dst, const std::vector<T> &h) {
+        hipError_t e = hipMemcpy(dst, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+        return e == hipSuccess ? 0 : fail(NEXD_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    }
+
+    template <typename T>
+    int upload(T **p, const std::vector<T> &h) {
+        if (!*p) {
+            int rc = dalloc(p, h.size());
+            if (rc) return rc;
+        }
+        return put(*p, h);
+    }
+
+    void drop_graph() {
+        if (graph) {
+            (void)hipGraphExecDestroy(graph);
+            graph = nullptr;
+        }
+    }
+
+    int nmat_sets() const { return cfg.shared_material ? 1 : cfg.nshot; }
+
+    int init() {
+        const nexd_batch_config &c = cfg;
+        np = (c.order + 1) * (c.order + 2) * (c.order + 3) / 6;
+        npf = (c.order + 1) * (c.order + 2) / 2;
+        nfp = 4 * npf;
+        ng = (size_t)c.nelem * np;
+        nrk = c.timeint == 1 ? 1 : (c.timeint == 2 ? 2 : 3);
+        if (hipSetDevice(c.device) != hipSuccess) return fail(NEXD_E_HIP, "hipSetDevice failed");
+        if (hipStreamCreate(&stream) != hipSuccess) return fail(NEXD_E_HIP, "hipStreamCreate failed");
+        if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess)
+            return fail(NEXD_E_HIP, "hipEventCreate failed");
+        const size_t S = (size_t)c.nshot;
+        int rc = 0;
+        for (int i = 0; i < 3 && !rc; ++i) rc = dalloc(&q[i], S * 9 * ng);
+        if (!rc && c.attenuation) (rc = dalloc(&th[0], S * 18 * ng)) || (rc = dalloc(&th[1], S * 18 * ng));
+        if (!rc && c.pml)
+            (rc = dalloc(&paux[0], S * 27 * ng)) || (rc = dalloc(&paux[1], S * 27 * ng)) ||
+                (rc = dalloc(&pmlc, (size_t)nmat_sets() * 9 * ng)) || (rc = dalloc(&last_flux, S * 27 * np));
+        if (!rc)
+            (rc = dalloc(&matc, (size_t)nmat_sets() * c.nelem * 8)) ||
+                (rc = dalloc(&anel, (size_t)nmat_sets() * c.nelem * nexd::NMB * 3)) ||
+                (rc = dalloc(&elist_plain, c.nelem)) || (rc = dalloc(&elist_pml, c.nelem)) || (rc = dalloc(&d_it, 1)) ||
+                (rc = dalloc(&seis, S * 9 * c.nt * (c.nrec > 0 ? c.nrec : 1))) ||
+                (rc = dalloc(&rec_slot, c.nelem)) || (rc = dalloc(&src_head, S * c.nelem)) ||
+                (rc = dalloc(&src_next, S * (c.nsrc > 0 ? c.nsrc : 1))) ||
+                (rc = dalloc(&srctype, S * (c.nsrc > 0 ? c.nsrc : 1))) ||
+                (rc = dalloc(&src_spatial, S * (c.nsrc > 0 ? c.nsrc : 1) * 6 * np)) ||
+                (rc = dalloc(&src_stf, S * (c.nsrc > 0 ? c.nsrc : 1) * c.nt));
+        if (rc) return rc;
+        pmlcheck.assign(c.nelem, 0);
+        have_mat.assign(nmat_sets(), 0);
+        have_att.assign(nmat_sets(), c.attenuation ? 0 : 1);
+        have_pml.assign(nmat_sets(), c.pml ? 0 : 1);
+        have_src.assign(c.nshot, c.nsrc ? 0 : 1);
+        have_rec = c.nrec == 0;
+        std::vector<int> minus(S * c.nelem, -1);
+        if ((rc = put(src_head, minus)) || (rc = put(rec_slot, std::vector<int>(c.nelem, -1)))) return rc;
+        return rebuild_lists();
+    }
+
+    int rebuild_lists() {
+        std::vector<int> plain, pmll;
+        const bool pml = cfg.pml != 0;
+        for (int e = 0; e < cfg.nelem; ++e) (pml && pmlcheck[e] ? pmll : plain).push_back(e);
+        n_plain = (int)plain.size();
+        n_pml = (int)pmll.size();
+        int rc;
+        if (n_plain && (rc = put(elist_plain, plain))) return rc;
+        if (n_pml && (rc = put(elist_pml, pmll))) return rc;
+        drop_graph();
+        return 0;
+    }
+
+    int check_shot(int shot) {
+        if (cfg.shared_material ? shot != 0 : (shot < 0 || shot >= cfg.nshot))
+            return fail(NEXD_E_ARG, "shot out of range (shared materials use shot 0)");
+        return 0;
+    }
+
+    nexd::StageArgs<R> base_args() {
+        const bool f32w = (cfg.compat & NEXD_COMPAT_RK3_FLOAT_WEIGHTS) != 0;
+        nexd::StageArgs<R> a{};
+        a.nglob = (int)ng;
+        a.nshot = cfg.nshot;
+        a.mat_stride = cfg.shared_material ? 0 : cfg.nelem * 8;
+        a.anel_stride = cfg.shared_material ? 0 : cfg.nelem * nexd::NMB * 3;
+        a.pmlc_stride_ng = cfg.shared_material ? 0 : 9;
+        a.src_factored = 1;
+        a.nt = cfg.nt;
+        a.nslot = nslot;
+        a.scheme = cfg.timeint;
+        a.dt = (R)cfg.dt;
+        a.w13 = f32w ? (R)(1.0f / 3.0f) : (R)(1.0 / 3.0);
+        a.w23 = f32w ? (R)(2.0f / 3.0f) : (R)(2.0 / 3.0);
+        a.pml_compat = (cfg.compat & NEXD_COMPAT_PML_LAST_ELEMENT) ? 1 : 0;
+        a.last_flux = last_flux;
+        a.D = D; a.LT = LT; a.fmask = fmask;
+        a.metric = metric; a.fscale = fscale; a.frame = frame; a.matc = matc;
+        a.nbr = nbr; a.ftype = ftype; a.anel = anel;
+        for (int i = 0; i < nexd::NMB; ++i) a.wl[i] = (R)wl[i];
+        a.pmlc = pmlc;
+        a.src_head = src_head; a.src_next = src_next; a.srctype = srctype;
+        a.src_spatial = src_spatial; a.src_stf = src_stf; a.nsrc = cfg.nsrc;
+        a.d_it = d_it;
+        a.rec_slot = rec_slot; a.u_slot = u_slot; a.a_slot = a_slot; a.u_full = nullptr;
+        a.blk_energy = nullptr;
+        a.mass = mass; a.jac = jac; a.emat = nullptr;
+        return a;
+    }
+
+    template <int ORDER, bool ATT, bool PML>
+    void launch_one(const nexd::StageArgs<R> &a) {
+        using Sh = nexd::Shape<ORDER, ATT, PML>;
+        const unsigned grid = (unsigned)((a.nlist + Sh::EB - 1) / Sh::EB) * (unsigned)a.nshot;
+        if (grid == 0) return;
+        hipLaunchKernelGGL((nexd::stage_kernel<ORDER, ATT, PML, R>), dim3(grid), dim3(Sh::NTHR), 0, stream, a);
+    }
+
+    template <int ORDER>
+    void stage(const nexd::StageArgs<R> &a, bool pml) {
+        const bool att = cfg.attenuation != 0;
+        if (att && pml) launch_one<ORDER, true, true>(a);
+        else if (att) launch_one<ORDER, true, false>(a);
+        else if (pml) launch_one<ORDER, false, true>(a);
+        else launch_one<ORDER, false, false>(a);
+    }
+
+    void dispatch(const nexd::StageArgs<R> &a, bool pml) {
+        switch (cfg.order) {
+            case 1: stage<1>(a, pml); break;
+            case 2: stage<2>(a, pml); break;
+            case 3: stage<3>(a, pml); break;
+            case 4: stage<4>(a, pml); break;
+            default: stage<5>(a, pml); break;
+        }
+    }
+
+    template <int ORDER>
+    void last_flux_impl(const R *qin, const R *pin) {
+        const int last = cfg.nelem - 1;
+        hipLaunchKernelGGL((nexd::last_flux_kernel<ORDER, R>), dim3(cfg.nshot), dim3(64), 0, stream, qin, pin, pmlc,
+                           matc, last, pmlcheck[last] ? 1 : 0, (int)ng, cfg.shared_material ? 0 : cfg.nelem * 8,
+                           cfg.shared_material ? 0 : 9, last_flux);
+    }
+
+    void last_flux_dispatch(const R *qin, const R *pin) {
+        switch (cfg.order) {
+            case 1: last_flux_impl<1>(qin, pin); break;
+            case 2: last_flux_impl<2>(qin, pin); break;
+            case 3: last_flux_impl<3>(qin, pin); break;
+            case 4: last_flux_impl<4>(qin, pin); break;
+            default: last_flux_impl<5>(qin, pin); break;
+        }
+    }
+
+    template <int ORDER>
+    void step_end_impl(int row_offset, int advance) {
+        hipLaunchKernelGGL((nexd::step_end_kernel<ORDER, R>), dim3(1), dim3(1024), 0, stream, q[0], u_slot, a_slot,
+                           rec_int, recelem, rec_slot_of, cfg.nrec, nslot, cfg.nshot, (int)ng, cfg.nt, row_offset,
+                           advance, seis, (const R *)nullptr, 0, (R *)nullptr, d_it);
+    }
+
+    void step_end(int row_offset, int advance) {
+        switch (cfg.order) {
+            case 1: step_end_impl<1>(row_offset, advance); break;
+            case 2: step_end_impl<2>(row_offset, advance); break;
+            case 3: step_end_impl<3>(row_offset, advance); break;
+            case 4: step_end_impl<4>(row_offset, advance); break;
+            default: step_end_impl<5>(row_offset, advance); break;
+        }
+    }
+
+    void set_stage_buffers(nexd::StageArgs<R> &a, int irk) {
+        const int scheme = cfg.timeint;
+        R *qin, *qout, *thin, *thout, *pin, *pout;
+        if (scheme == NEXD_EULER) {
+            qin = q[0]; qout = q[1]; thin = th[0]; thout = th[0]; pin = paux[0]; pout = paux[0];
+        } else if (scheme == NEXD_RK2) {
+            if (irk == 1) { qin = q[0]; qout = q[1]; thin = th[0]; thout = th[1]; pin = paux[0]; pout = paux[1]; }
+            else { qin = q[1]; qout = q[0]; thin = th[1]; thout = th[0]; pin = paux[1]; pout = paux[0]; }
+        } else {
+            if (irk == 1) { qin = q[0]; qout = q[1]; thin = th[0]; thout = th[1]; pin = paux[0]; pout = paux[1]; }
+            else if (irk == 2) { qin = q[1]; qout = q[2]; thin = th[1]; thout = th[1]; pin = paux[1]; pout = paux[1]; }
+            else { qin = q[2]; qout = q[0]; thin = th[1]; thout = th[0]; pin = paux[1]; pout = paux[0]; }
+        }
+        a.irk = irk;
+        a.last_stage = irk == nrk ? 1 : 0;
+        a.qin = qin; a.qbase = q[0]; a.qout = qout;
+        a.thin = thin; a.thbase = th[0]; a.thout = thout;
+        a.pin = pin; a.pbase = paux[0]; a.pout = pout;
+    }
+
+    void enqueue_step() {
+        nexd::StageArgs<R> a = base_args();
+        const bool pml = cfg.pml && n_pml > 0;
+        for (int irk = 1; irk <= nrk; ++irk) {
+            set_stage_buffers(a, irk);
+            if (pml && a.pml_compat) last_flux_dispatch(a.qin, a.pin);
+            a.elist = elist_plain;
+            a.nlist = n_plain;
+            dispatch(a, false);
+            if (pml) {
+                a.elist = elist_pml;
+                a.nlist = n_pml;
+                dispatch(a, true);
+            }
+            if (cfg.timeint == NEXD_EULER)
+                (void)hipMemcpyAsync(q[0], q[1], sizeof(R) * 9 * ng * cfg.nshot, hipMemcpyDeviceToDevice, stream);
+        }
+        step_end(1, 1);
+    }
+
+    bool ready() const {
+        for (char c : have_mat) if (!c) return false;
+        for (char c : have_att) if (!c) return false;
+        for (char c : have_pml) if (!c) return false;
+        for (char c : have_src) if (!c) return false;
+        return have_ops && have_mesh && have_rec;
+    }
+
+    int reset() {
+        const size_t S = cfg.nshot;
+        const R tiny = (R)(double)1e-24f;   // the reference's default-REAL 1e-24
+        for (int i = 0; i < 3; ++i) {
+            int rc = put(q[i], std::vector<R>(S * 9 * ng, tiny));
+            if (rc) return rc;
+        }
+        if (cfg.attenuation)
+            for (int i = 0; i < 2; ++i) {
+                int rc = put(th[i], std::vector<R>(S * 18 * ng, tiny));
+                if (rc) return rc;
+            }
+        if (cfg.pml)
+            for (int i = 0; i < 2; ++i) {
+                int rc = put(paux[i], std::vector<R>(S * 27 * ng, R(0)));
+                if (rc) return rc;
+            }
+        if (nslot) {
+            std::vector<R> t(S * nslot * 3 * np, tiny);
+            int rc;
+            if ((rc = put(u_slot, t)) || (rc = put(a_slot, t))) return rc;
+        }
+        int rc = put(seis, std::vector<R>(S * 9 * cfg.nt * (cfg.nrec > 0 ? cfg.nrec : 1), R(0)));
+        if (rc) return rc;
+        if ((rc = put(d_it, std::vector<int>(1, 0)))) return rc;
+        step = 0;
+        initialized = true;
+        return 0;
+    }
+
+    int run(int it_begin, int it_end) {
+        if (!ready()) return fail(NEXD_E_STATE, "upload operators, mesh, materials, PML, sources and receivers first");
+        if (!initialized) {
+            int rc = reset();
+            if (rc) return rc;
+        }
+        if (it_begin != step + 1) return fail(NEXD_E_ARG, "it_begin must be the next step");
+        if (it_end < it_begin || it_end > cfg.nt) return fail(NEXD_E_ARG, "it_end out of range");
+        if (hipSetDevice(cfg.device) != hipSuccess) return fail(NEXD_E_HIP, "hipSetDevice failed");
+        if (step == 0) step_end(0, 0);
+        if (!graph) {
+            hipGraph_t g = nullptr;
+            if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                return fail(NEXD_E_HIP, "hipStreamBeginCapture failed");
+            enqueue_step();
+            if (hipStreamEndCapture(stream, &g) != hipSuccess) return fail(NEXD_E_HIP, "hipStreamEndCapture failed");
+            hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                graph = nullptr;
+                return fail(NEXD_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            }
+        }
+        if (hipEventRecord(ev0, stream) != hipSuccess) return fail(NEXD_E_HIP, "hipEventRecord failed");
+        for (int it = it_begin; it <= it_end; ++it)
+            if (hipGraphLaunch(graph, stream) != hipSuccess) return fail(NEXD_E_HIP, "hipGraphLaunch failed");
+        if (hipEventRecord(ev1, stream) != hipSuccess) return fail(NEXD_E_HIP, "hipEventRecord failed");
+        hipError_t e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return fail(NEXD_E_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+        e = hipGetLastError();
+        if (e != hipSuccess) return fail(NEXD_E_HIP, std::string("kernel: ") + hipGetErrorString(e));
+        (void)hipEventElapsedTime(&last_ms, ev0, ev1);
+        step = it_end;
+        return 0;
+    }
+
+    int time_stage(int list, int nrep, double *avg_ms, int *nel) {
+        if (!initialized) return fail(NEXD_E_STATE, "run or reset before timing");
+        if (nrk == 1) return fail(NEXD_E_UNSUPPORTED, "timing needs a multi-stage scheme");
+        const bool pml = cfg.pml && n_pml > 0;
+        if (list == 1 && !pml) return fail(NEXD_E_STATE, "no PML element list");
+        nexd::StageArgs<R> a = base_args();
+        set_stage_buffers(a, 1);
+        a.elist = list == 0 ? elist_plain : elist_pml;
+        a.nlist = list == 0 ? n_plain : n_pml;
+        if (list == 1 && a.pml_compat) last_flux_dispatch(a.qin, a.pin);
+        dispatch(a, list == 1);
+        if (hipEventRecord(ev0, stream) != hipSuccess) return fail(NEXD_E_HIP, "hipEventRecord failed");
+        for (int r = 0; r < nrep; ++r) dispatch(a, list == 1);
+        if (hipEventRecord(ev1, stream) != hipSuccess) return fail(NEXD_E_HIP, "hipEventRecord failed");
+        if (hipStreamSynchronize(stream) != hipSuccess) return fail(NEXD_E_HIP, "hipStreamSynchronize failed");
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, ev0, ev1);
+        *avg_ms = ms / nrep;
+        if (nel) *nel = a.nlist;
+        return 0;
+    }
+};
+
+template <typename R>
+Batch<R> *as(nexd_batch *b) {
+    return static_cast<Batch<R> *>(reinterpret_cast<BatchBase *>(b));
+}
+
+BatchBase *base(nexd_batch *b) { return reinterpret_cast<BatchBase *>(b); }
+
+#define DISPATCH(b, call)                                                                   \
+    (base(b)->cfg.precision == NEXD_FP32 ? as<float>(b)->call : as<double>(b)->call)
+
+template <typename R>
+int upload_ops(Batch<R> *b, const double *dr, const double *ds, const double *dt, const double *lift,
+               const double *mass, const int *fmask) {
+    nexd::OpsHost o;
+    const std::string err = nexd::convert_operators(b->np, b->npf, dr, ds, dt, lift, mass, fmask, o);
+    if (!err.empty()) return b->fail(NEXD_E_ARG, err);
+    int rc;
+    if ((rc = b->upload(&b->D, cvt<R>(o.D))) || (rc = b->upload(&b->LT, cvt<R>(o.LT))) ||
+        (rc = b->upload(&b->mass, cvt<R>(o.M))) || (rc = b->upload(&b->fmask, o.fm)))
+        return rc;
+    b->fmask_host = o.fm;
+    b->have_ops = true;
+    b->drop_graph();
+    return 0;
+}
+
+template <typename R>
+int upload_mesh(Batch<R> *b, const int *ibool, const double *const met[9], const double *jac, const double *fscale,
+                const int *neighbor, const int *face, const int *ibt, const int *ibn, const double *nn,
+                const double *st, const double *tt) {
+    if (!b->have_ops) return b->fail(NEXD_E_STATE, "upload operators before the mesh");
+    nexd::MeshHost m;
+    const std::string err = nexd::convert_mesh(b->cfg.nelem, b->np, b->npf, b->fmask_host, ibool, met, jac, fscale,
+                                               neighbor, face, ibt, ibn, nn, st, tt, m);
+    if (!err.empty())
+        return err[0] == '!' ? b->fail(NEXD_E_UNSUPPORTED, err.substr(1)) : b->fail(NEXD_E_ARG, err);
+    int rc;
+    if ((rc = b->upload(&b->metric, cvt<R>(m.metric))) || (rc = b->upload(&b->jac, cvt<R>(m.jac))) ||
+        (rc = b->upload(&b->fscale, cvt<R>(m.fscale))) || (rc = b->upload(&b->frame, cvt<R>(m.frame))) ||
+        (rc = b->upload(&b->nbr, m.nbr)) || (rc = b->upload(&b->ftype, m.ftype)))
+        return rc;
+    b->have_mesh = true;
+    b->drop_graph();
+    return 0;
+}
+
+template <typename R>
+int upload_material(Batch<R> *b, int shot, const double *vpu, const double *vsu, const double *rho,
+                    const double *lambdau, const double *muu) {
+    int rc = b->check_shot(shot);
+    if (rc) return rc;
+    if (!vpu || !vsu || !rho || !lambdau || !muu) return b->fail(NEXD_E_ARG, "null material");
+    const int nel = b->cfg.nelem;
+    std::vector<R> mc((size_t)nel * 8, R(0));
+    for (int e = 0; e < nel; ++e) {
+        if (!(rho[e] > 0)) return b->fail(NEXD_E_ARG, "density must be positive");
+        mc[e * 8 + 0] = (R)vpu[e]; mc[e * 8 + 1] = (R)vsu[e]; mc[e * 8 + 2] = (R)rho[e];
+        mc[e * 8 + 3] = (R)lambdau[e]; mc[e * 8 + 4] = (R)muu[e];
+    }
+    if ((rc = b->put(b->matc + (size_t)shot * nel * 8, mc))) return rc;
+    b->have_mat[shot] = 1;
+    b->drop_graph();
+    return 0;
+}
+
+template <typename R>
+int upload_anelastic(Batch<R> *b, int shot, const double *wl, const double *av) {
+    int rc = b->check_shot(shot);
+    if (rc) return rc;
+    if (!b->cfg.attenuation) return b->fail(NEXD_E_STATE, "batch created without attenuation");
+    if (!wl || !av) return b->fail(NEXD_E_ARG, "null anelastic array");
+    bool any_before = false;
+    for (char c : b->have_att) any_before |= c != 0;
+    for (int i = 0; i < nexd::NMB; ++i)
+        if (any_before && b->wl[i] != wl[i])
+            return b->fail(NEXD_E_ARG, "relaxation frequencies must be the same for every shot");
+    for (int i = 0; i < nexd::NMB; ++i) b->wl[i] = wl[i];
+    const size_t n = (size_t)b->cfg.nelem * nexd::NMB * 3;
+    std::vector<R> v(av, av + n);
+    if ((rc = b->put(b->anel + (size_t)shot * n, v))) return rc;
+    b->have_att[shot] = 1;
+    b->drop_graph();
+    return 0;
+}
+
+template <typename R>
+int upload_pml(Batch<R> *b, int shot, const int *pmlcheck, const double *const prof[9]) {
+    int rc = b->check_shot(shot);
+    if (rc) return rc;
+    if (!b->cfg.pml) return b->fail(NEXD_E_STATE, "batch created without PML");
+    if (!pmlcheck) return b->fail(NEXD_E_ARG, "null PML array");
+    for (int k = 0; k < 9; ++k)
+        if (!prof[k]) return b->fail(NEXD_E_ARG, "null PML array");
+    bool changed = false;
+    for (int e = 0; e < b->cfg.nelem; ++e) {
+        const int v = pmlcheck[e] ? 1 : 0;
+        if (v != b->pmlcheck[e]) changed = true;
+        b->pmlcheck[e] = v;
+    }
+    bool any_before = false;
+    for (char c : b->have_pml) any_before |= c != 0;
+    if (changed && any_before) return b->fail(NEXD_E_ARG, "pmlcheck must be the same for every shot");
+    std::vector<double> pc;
+    nexd::pml_coefficients(b->ng, prof[0], prof[1], prof[2], prof[3], prof[4], prof[5], prof[6], prof[7], prof[8], pc);
+    if ((rc = b->put(b->pmlc + (size_t)shot * 9 * b->ng, cvt<R>(pc)))) return rc;
+    b->have_pml[shot] = 1;
+    return b->rebuild_lists();
+}
+
+template <typename R>
+int upload_sources(Batch<R> *b, int shot, const int *srcelem, const int *srctype, const double *spatial,
+                   const double *stf) {
+    const int ns = b->cfg.nsrc, nel = b->cfg.nelem, np = b->np, nt = b->cfg.nt;
+    if (shot < 0 || shot >= b->cfg.nshot) return b->fail(NEXD_E_ARG, "shot out of range");
+    if (ns == 0) { b->have_src[shot] = 1; return 0; }
+    if (!srcelem || !srctype || !spatial || !stf) return b->fail(NEXD_E_ARG, "null source array");
+    std::vector<int> head(nel, -1), next(ns, -1), st(ns);
+    for (int s = ns - 1; s >= 0; --s) {
+        const int e = srcelem[s] - 1;
+        if (e < 0 || e >= nel) return b->fail(NEXD_E_ARG, "srcelem out of range");
+        if (srctype[s] != 0 && srctype[s] != 1) return b->fail(NEXD_E_ARG, "srctype must be 0 or 1");
+        st[s] = srctype[s];
+        next[s] = head[e];
+        head[e] = s;
+    }
+    std::vector<R> sp(spatial, spatial + (size_t)ns * 6 * np);
+    // stf arrives as (nt,nsrc) column-major, i.e. [nsrc][nt] in memory
+    std::vector<R> sf((size_t)ns * nt);
+    for (int s = 0; s < ns; ++s)
+        for (int it = 0; it < nt; ++it) sf[(size_t)s * nt + it] = (R)stf[(size_t)s * nt + it];
+    int rc;
+    if ((rc = b->put(b->src_head + (size_t)shot * nel, head)) || (rc = b->put(b->src_next + (size_t)shot * ns, next)) ||
+        (rc = b->put(b->srctype + (size_t)shot * ns, st)) ||
+        (rc = b->put(b->src_spatial + (size_t)shot * ns * 6 * np, sp)) ||
+        (rc = b->put(b->src_stf + (size_t)shot * ns * nt, sf)))
+        return rc;
+    b->have_src[shot] = 1;
+    b->drop_graph();
+    return 0;
+}
+
+template <typename R>
+int upload_receivers(Batch<R> *b, const int *recelem, const double *rec_int) {
+    const int nr = b->cfg.nrec, np = b->np, nel = b->cfg.nelem;
+    if (nr == 0) { b->have_rec = true; return 0; }
+    if (!recelem || !rec_int) return b->fail(NEXD_E_ARG, "null receiver array");
+    if (b->u_slot) return b->fail(NEXD_E_STATE, "receivers already uploaded");
+    std::vector<int> slot_of_elem(nel, -1), slot_of_rec(nr), re(nr);
+    int nslot = 0;
+    for (int r = 0; r < nr; ++r) {
+        const int e = recelem[r] - 1;
+        if (e < 0 || e >= nel) return b->fail(NEXD_E_ARG, "recelem out of range");
+        if (slot_of_elem[e] < 0) slot_of_elem[e] = nslot++;
+        slot_of_rec[r] = slot_of_elem[e];
+        re[r] = e;
+    }
+    int rc;
+    const size_t S = b->cfg.nshot;
+    if ((rc = b->dalloc(&b->u_slot, S * nslot * 3 * np)) || (rc = b->dalloc(&b->a_slot, S * nslot * 3 * np))) return rc;
+    b->nslot = nslot;
+    std::vector<R> w(rec_int, rec_int + (size_t)np * nr);
+    if ((rc = b->put(b->rec_slot, slot_of_elem)) || (rc = b->upload(&b->rec_slot_of, slot_of_rec)) ||
+        (rc = b->upload(&b->recelem, re)) || (rc = b->upload(&b->rec_int, w)))
+        return rc;
+    b->have_rec = true;
+    b->drop_graph();
+    return 0;
+}
+
+template <typename R>
+int get_seis(Batch<R> *b, int shot, double *const outs[9]) {
+    if (shot < 0 || shot >= b->cfg.nshot) return b->fail(NEXD_E_ARG, "shot out of range");
+    const size_t n = (size_t)b->cfg.nt * b->cfg.nrec;
+    std::vector<R> h(n);
+    for (int k = 0; k < 9; ++k) {
+        if (!outs[k]) continue;
+        hipError_t e = hipMemcpy(h.data(), b->seis + ((size_t)shot * 9 + k) * n, sizeof(R) * n, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return b->fail(NEXD_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+        for (size_t i = 0; i < n; ++i) outs[k][i] = (double)h[i];
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nexd_batch_create(const nexd_batch_config *cfg, nexd_batch **out) {
+    if (!cfg || !out) { g_batch_create_error = "null argument"; return NEXD_E_ARG; }
+    *out = nullptr;
+    if (cfg->order < 1 || cfg->order > 5) { g_batch_create_error = "order must be 1..5"; return NEXD_E_UNSUPPORTED; }
+    if (cfg->nelem <= 0 || cfg->nt <= 0 || cfg->nshot <= 0 || cfg->nsrc < 0 || cfg->nrec < 0) {
+        g_batch_create_error = "nelem, nt, nshot must be positive; nsrc, nrec non-negative";
+        return NEXD_E_ARG;
+    }
+    if (cfg->timeint < 1 || cfg->timeint > 3) { g_batch_create_error = "timeint must be 1, 2 or 3"; return NEXD_E_UNSUPPORTED; }
+    if (cfg->precision != NEXD_FP32 && cfg->precision != NEXD_FP64) {
+        g_batch_create_error = "precision must be NEXD_FP32 or NEXD_FP64";
+        return NEXD_E_ARG;
+    }
+    if (!(cfg->dt > 0)) { g_batch_create_error = "dt must be positive"; return NEXD_E_ARG; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { g_batch_create_error = "no HIP device"; return NEXD_E_HIP; }
+    if (cfg->device < 0 || cfg->device >= ndev) { g_batch_create_error = "device ordinal out of range"; return NEXD_E_ARG; }
+    BatchBase *b;
+    int rc;
+    if (cfg->precision == NEXD_FP32) {
+        auto *p = new Batch<float>();
+        p->cfg = *cfg;
+        rc = p->init();
+        b = p;
+    } else {
+        auto *p = new Batch<double>();
+        p->cfg = *cfg;
+        rc = p->init();
+        b = p;
+    }
+    if (rc) {
+        g_batch_create_error = b->err;
+        delete b;
+        return rc;
+    }
+    *out = reinterpret_cast<nexd_batch *>(b);
+    return NEXD_OK;
+}
+
+int nexd_batch_upload_operators(nexd_batch *b, const double *dr, const double *ds, const double *dt,
+                                const double *lift, const double *mass, const int *fmask) {
+    if (!b) return NEXD_E_ARG;
+    return base(b)->cfg.precision == NEXD_FP32 ? upload_ops(as<float>(b), dr, ds, dt, lift, mass, fmask)
+                                               : upload_ops(as<double>(b), dr, ds, dt, lift, mass, fmask);
+}
+
+int nexd_batch_upload_mesh(nexd_batch *b, const int *ibool, const double *rx, const double *ry, const double *rz,
+                           const double *sx, const double *sy, const double *sz, const double *tx, const double *ty,
+                           const double *tz, const double *jacobian, const double *fscale, const int *neighbor,
+                           const int *face, const int *ibt, const int *ibn, const double *nnormal,
+                           const double *stangential, const double *ttangential) {
+    if (!b) return NEXD_E_ARG;
+    const double *met[9] = {rx, ry, rz, sx, sy, sz, tx, ty, tz};
+    return base(b)->cfg.precision == NEXD_FP32
+               ? upload_mesh(as<float>(b), ibool, met, jacobian, fscale, neighbor, face, ibt, ibn, nnormal, stangential, ttangential)
+               : upload_mesh(as<double>(b), ibool, met, jacobian, fscale, neighbor, face, ibt, ibn, nnormal, stangential, ttangential);
+}
+
+int nexd_batch_upload_material(nexd_batch *b, int shot, const double *vpu, const double *vsu, const double *rho,
+                               const double *lambdau, const double *muu) {
+    if (!b) return NEXD_E_ARG;
+    return base(b)->cfg.precision == NEXD_FP32 ? upload_material(as<float>(b), shot, vpu, vsu, rho, lambdau, muu)
+                                               : upload_material(as<double>(b), shot, vpu, vsu, rho, lambdau, muu);
+}
+
+int nexd_batch_upload_anelastic(nexd_batch *b, int shot, const double *wl, const double *anelasticvar) {
+    if (!b) return NEXD_E_ARG;
+    return base(b)->cfg.precision == NEXD_FP32 ? upload_anelastic(as<float>(b), shot, wl, anelasticvar)
+                                               : upload_anelastic(as<double>(b), shot, wl, anelasticvar);
+}
+
+int nexd_batch_upload_pml(nexd_batch *b, int shot, const int *pmlcheck, const double *ddx, const double *ddy,
+                          const double *ddz, const double *alphax, const double *alphay, const double *alphaz,
+                          const double *kx, const double *ky, const double *kz) {
+    if (!b) return NEXD_E_ARG;
+    const double *prof[9] = {ddx, ddy, ddz, alphax, alphay, alphaz, kx, ky, kz};
+    return base(b)->cfg.precision == NEXD_FP32 ? upload_pml(as<float>(b), shot, pmlcheck, prof)
+                                               : upload_pml(as<double>(b), shot, pmlcheck, prof);
+}
+
+int nexd_batch_upload_sources(nexd_batch *b, int shot, const int *srcelem, const int *srctype, const double *spatial,
+                              const double *stf) {
+    if (!b) return NEXD_E_ARG;
+    return base(b)->cfg.precision == NEXD_FP32 ? upload_sources(as<float>(b), shot, srcelem, srctype, spatial, stf)
+                                               : upload_sources(as<double>(b), shot, srcelem, srctype, spatial, stf);
+}
+
+int nexd_batch_upload_receivers(nexd_batch *b, const int *recelem, const double *rec_int) {
+    if (!b) return NEXD_E_ARG;
+    return base(b)->cfg.precision == NEXD_FP32 ? upload_receivers(as<float>(b), recelem, rec_int)
+                                               : upload_receivers(as<double>(b), recelem, rec_int);
+}
+
+int nexd_batch_reset(nexd_batch *b) {
+    if (!b) return NEXD_E_ARG;
+    return DISPATCH(b, reset());
+}
+
+int nexd_batch_run(nexd_batch *b, int it_begin, int it_end) {
+    if (!b) return NEXD_E_ARG;
+    return DISPATCH(b, run(it_begin, it_end));
+}
+
+int nexd_batch_get_seismograms(nexd_batch *b, int shot, double *ux, double *uy, double *uz, double *vx, double *vy,
+                               double *vz, double *ax, double *ay, double *az) {
+    if (!b) return NEXD_E_ARG;
+    double *const outs[9] = {ux, uy, uz, vx, vy, vz, ax, ay, az};
+    return base(b)->cfg.precision == NEXD_FP32 ? get_seis(as<float>(b), shot, outs) : get_seis(as<double>(b), shot, outs);
+}
+
+int nexd_batch_last_run_ms(nexd_batch *b, double *ms) {
+    if (!b || !ms) return NEXD_E_ARG;
+    *ms = DISPATCH(b, last_ms);
+    return 0;
+}
+
+int nexd_batch_time_stage_kernel(nexd_batch *b, int list, int nrep, double *avg_ms, int *nelem_launch) {
+    if (!b || !avg_ms || nrep <= 0 || (list != 0 && list != 1)) return NEXD_E_ARG;
+    return DISPATCH(b, time_stage(list, nrep, avg_ms, nelem_launch));
+}
+
+void nexd_batch_destroy(nexd_batch *b) {
+    if (b) delete base(b);
+}
+
+const char *nexd_batch_last_error(const nexd_batch *b) {
+    return b ? reinterpret_cast<const BatchBase *>(b)->err.c_str() : g_batch_create_error.c_str();
+}
+
+}  // extern "C"

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/nexd_gpu.h"
+#include "host_layout.hpp"
 #include "plain_kernels.hpp"
 
 using nexd::Dims;
@@ -45,6 +46,7 @@ struct nexd_ctx {
     // operators and mesh
     double *D = nullptr, *LT = nullptr, *mass = nullptr;
     int *fmask = nullptr;
+    std::vector<int> fmask_host;
     double *metric = nullptr, *fscale = nullptr, *frame = nullptr, *jac = nullptr;
     int *nbr = nullptr, *ftype = nullptr;
     double *matc = nullptr, *emat = nullptr, *anel = nullptr, *pmlc = nullptr;
@@ -121,7 +123,7 @@ void drop_graph(nexd_ctx *c) {
 template <int ORDER, bool ATT, bool PML>
 void launch_one(const StageArgs<double> &a, hipStream_t s) {
     using S = nexd::Shape<ORDER, ATT, PML>;
-    const int grid = (a.nlist + S::EB - 1) / S::EB;
+    const int grid = (a.nlist + S::EB - 1) / S::EB * a.nshot;
     if (grid == 0) return;
     hipLaunchKernelGGL((nexd::stage_kernel<ORDER, ATT, PML, double>), dim3(grid), dim3(S::NTHR), 0, s, a);
 }
@@ -168,7 +170,7 @@ template <int ORDER>
 void launch_last_flux(nexd_ctx *c, const double *qin, const double *pin) {
     const int last = c->cfg.nelem - 1;
     hipLaunchKernelGGL((nexd::last_flux_kernel<ORDER, double>), dim3(1), dim3(64), 0, c->stream, qin, pin, c->pmlc,
-                       c->matc, last, c->pmlcheck[last] ? 1 : 0, (int)c->nglob, c->last_flux);
+                       c->matc, last, c->pmlcheck[last] ? 1 : 0, (int)c->nglob, 0, 0, c->last_flux);
 }
 
 void dispatch_last_flux(nexd_ctx *c, const double *qin, const double *pin) {
@@ -185,8 +187,8 @@ template <int ORDER>
 void launch_step_end(nexd_ctx *c, int row_offset, int advance) {
     const double *be = (c->cfg.energy && c->pml_active && c->have_pml) ? c->blk_energy : nullptr;
     hipLaunchKernelGGL((nexd::step_end_kernel<ORDER, double>), dim3(1), dim3(1024), 0, c->stream, c->q[0], c->u_slot,
-                       c->a_slot, c->rec_int, c->recelem, c->rec_slot_of, c->cfg.nrec, (int)c->nglob, c->cfg.nt,
-                       row_offset, advance, c->seis, be, c->nblk_total, c->energy, c->d_it);
+                       c->a_slot, c->rec_int, c->recelem, c->rec_slot_of, c->cfg.nrec, c->nslot, 1, (int)c->nglob,
+                       c->cfg.nt, row_offset, advance, c->seis, be, c->nblk_total, c->energy, c->d_it);
 }
 
 void dispatch_step_end(nexd_ctx *c, int row_offset, int advance) {
@@ -203,6 +205,13 @@ StageArgs<double> base_args(nexd_ctx *c) {
     const bool f32w = (c->cfg.compat & NEXD_COMPAT_RK3_FLOAT_WEIGHTS) != 0;
     StageArgs<double> a{};
     a.nglob = (int)c->nglob;
+    a.nshot = 1;
+    a.mat_stride = 0;
+    a.anel_stride = 0;
+    a.pmlc_stride_ng = 0;
+    a.src_factored = 0;
+    a.nt = c->cfg.nt;
+    a.nslot = c->nslot;
     a.scheme = c->cfg.timeint;
     a.dt = c->cfg.dt;
     a.w13 = f32w ? (double)(1.0f / 3.0f) : 1.0 / 3.0;
@@ -353,28 +362,15 @@ int nexd_create(const nexd_config *cfg, nexd_ctx **out) {
 
 int nexd_upload_operators(nexd_ctx *c, const double *dr, const double *ds, const double *dt, const double *lift,
                           const double *mass, const int *fmask) {
-    if (!c || !dr || !ds || !dt || !lift || !mass || !fmask) return fail(c, NEXD_E_ARG, "null operator");
-    const int np = c->np, npf = c->npf, nfp = c->nfp;
-    std::vector<double> D(3 * np * np), LT(nfp * np), M(np * np);
-    const double *src[3] = {dr, ds, dt};
-    for (int d = 0; d < 3; ++d)
-        for (int i = 0; i < np; ++i)
-            for (int m = 0; m < np; ++m) D[(d * np + i) * np + m] = src[d][(size_t)m * np + i];
-    for (int k = 0; k < nfp; ++k)
-        for (int j = 0; j < np; ++j) LT[k * np + j] = lift[(size_t)k * np + j];
-    for (int i = 0; i < np; ++i)
-        for (int k = 0; k < np; ++k) M[i * np + k] = mass[(size_t)k * np + i];
-    std::vector<int> fm(4 * npf);
-    for (int f = 0; f < 4; ++f)
-        for (int i = 0; i < npf; ++i) {
-            int v = fmask[f * npf + i] - 1;
-            if (v < 0 || v >= np) return fail(c, NEXD_E_ARG, "fmask entry out of range");
-            fm[f * npf + i] = v;
-        }
+    if (!c) return fail(c, NEXD_E_ARG, "null context");
+    nexd::OpsHost o;
+    const std::string err = nexd::convert_operators(c->np, c->npf, dr, ds, dt, lift, mass, fmask, o);
+    if (!err.empty()) return fail(c, NEXD_E_ARG, err);
     int rc;
-    if ((rc = upload(c, &c->D, D)) || (rc = upload(c, &c->LT, LT)) || (rc = upload(c, &c->mass, M)) ||
-        (rc = upload(c, &c->fmask, fm)))
+    if ((rc = upload(c, &c->D, o.D)) || (rc = upload(c, &c->LT, o.LT)) || (rc = upload(c, &c->mass, o.M)) ||
+        (rc = upload(c, &c->fmask, o.fm)))
         return rc;
+    c->fmask_host = o.fm;
     c->have_ops = true;
     drop_graph(c);
     return NEXD_OK;
@@ -385,57 +381,19 @@ int nexd_upload_mesh(nexd_ctx *c, const int *ibool, const double *rx, const doub
                      const double *tz, const double *jacobian, const double *fscale, const int *neighbor,
                      const int *face, const int *ibt, const int *ibn, const double *nnormal,
                      const double *stangential, const double *ttangential) {
-    if (!c || !ibool || !rx || !ry || !rz || !sx || !sy || !sz || !tx || !ty || !tz || !jacobian || !fscale ||
-        !neighbor || !face || !ibt || !ibn || !nnormal || !stangential || !ttangential)
-        return fail(c, NEXD_E_ARG, "null mesh array");
+    if (!c) return fail(c, NEXD_E_ARG, "null context");
     if (!c->have_ops) return fail(c, NEXD_E_STATE, "upload operators before the mesh");
-    const int np = c->np, npf = c->npf, nfp = c->nfp, nel = c->cfg.nelem;
-    const size_t ng = c->nglob;
-    for (size_t i = 0; i < ng; ++i)
-        if (ibool[i] != (int)i + 1) return fail(c, NEXD_E_UNSUPPORTED, "ibool must be (ie-1)*Np+j");
-    std::vector<int> fm(4 * npf);
-    HIPCHK(c, hipMemcpy(fm.data(), c->fmask, sizeof(int) * fm.size(), hipMemcpyDeviceToHost));
-    std::vector<double> met(9 * ng);
-    const double *ms[9] = {rx, ry, rz, sx, sy, sz, tx, ty, tz};
-    for (int k = 0; k < 9; ++k) std::memcpy(met.data() + k * ng, ms[k], sizeof(double) * ng);
-    std::vector<double> jac(jacobian, jacobian + ng);
-    std::vector<double> fs(fscale, fscale + (size_t)nel * nfp);
-    std::vector<double> fr((size_t)nel * 36);
-    std::vector<int> nb((size_t)nel * 4 * npf), ft((size_t)nel * 4);
-    for (int e = 0; e < nel; ++e) {
-        for (int f = 0; f < 4; ++f) {
-            for (int k = 0; k < 3; ++k) {
-                fr[(e * 4 + f) * 9 + k] = nnormal[(e * 4 + f) * 3 + k];
-                fr[(e * 4 + f) * 9 + 3 + k] = stangential[(e * 4 + f) * 3 + k];
-                fr[(e * 4 + f) * 9 + 6 + k] = ttangential[(e * 4 + f) * 3 + k];
-            }
-            const int n = neighbor[e * 4 + f], fc = face[e * 4 + f];
-            for (int i = 0; i < npf; ++i) {
-                const size_t idx = ((size_t)e * 4 + f) * npf + i;
-                if ((size_t)ibt[idx] != (size_t)e * np + fm[f * npf + i] + 1)
-                    return fail(c, NEXD_E_UNSUPPORTED, "ibt must be ibool(fmask(:,is),ie)");
-            }
-            if (n > 0) {
-                if (fc < 1 || fc > 4) return fail(c, NEXD_E_ARG, "face of an interior face must be 1..4");
-                ft[e * 4 + f] = fc - 1;
-                for (int i = 0; i < npf; ++i) {
-                    const size_t idx = ((size_t)e * 4 + f) * npf + i;
-                    const int v = ibn[idx] - 1;
-                    if (v < 0 || (size_t)v >= ng) return fail(c, NEXD_E_ARG, "ibn out of range");
-                    nb[idx] = v;
-                }
-            } else if (n == 0) {
-                if (fc != -1 && fc != -2) return fail(c, NEXD_E_ARG, "boundary face must be free (-1) or absorbing (-2)");
-                ft[e * 4 + f] = fc;
-                for (int i = 0; i < npf; ++i) nb[((size_t)e * 4 + f) * npf + i] = -1;
-            } else {
-                return fail(c, NEXD_E_UNSUPPORTED, "partition interfaces (neighbor == -1) are not supported");
-            }
-        }
-    }
+    const double *met[9] = {rx, ry, rz, sx, sy, sz, tx, ty, tz};
+    nexd::MeshHost m;
+    const std::string err = nexd::convert_mesh(c->cfg.nelem, c->np, c->npf, c->fmask_host, ibool, met, jacobian,
+                                               fscale, neighbor, face, ibt, ibn, nnormal, stangential,
+                                               ttangential, m);
+    if (!err.empty())
+        return err[0] == '!' ? fail(c, NEXD_E_UNSUPPORTED, err.substr(1)) : fail(c, NEXD_E_ARG, err);
     int rc;
-    if ((rc = upload(c, &c->metric, met)) || (rc = upload(c, &c->jac, jac)) || (rc = upload(c, &c->fscale, fs)) ||
-        (rc = upload(c, &c->frame, fr)) || (rc = upload(c, &c->nbr, nb)) || (rc = upload(c, &c->ftype, ft)))
+    if ((rc = upload(c, &c->metric, m.metric)) || (rc = upload(c, &c->jac, m.jac)) ||
+        (rc = upload(c, &c->fscale, m.fscale)) || (rc = upload(c, &c->frame, m.frame)) ||
+        (rc = upload(c, &c->nbr, m.nbr)) || (rc = upload(c, &c->ftype, m.ftype)))
         return rc;
     c->have_mesh = true;
     drop_graph(c);
@@ -478,13 +436,8 @@ int nexd_upload_pml(nexd_ctx *c, const int *pmlcheck, const double *ddx, const d
     if (!c || !pmlcheck || !ddx || !ddy || !ddz || !alphax || !alphay || !alphaz || !kx || !ky || !kz)
         return fail(c, NEXD_E_ARG, "null PML array");
     if (!c->cfg.pml) return fail(c, NEXD_E_STATE, "context created without PML");
-    const size_t ng = c->nglob;
-    std::vector<double> pc(9 * ng);
-    for (size_t g = 0; g < ng; ++g) {
-        pc[0 * ng + g] = alphax[g]; pc[1 * ng + g] = alphay[g]; pc[2 * ng + g] = alphaz[g];
-        pc[3 * ng + g] = ddx[g] / kx[g]; pc[4 * ng + g] = ddy[g] / ky[g]; pc[5 * ng + g] = ddz[g] / kz[g];
-        pc[6 * ng + g] = kx[g]; pc[7 * ng + g] = ky[g]; pc[8 * ng + g] = kz[g];
-    }
+    std::vector<double> pc;
+    nexd::pml_coefficients(c->nglob, ddx, ddy, ddz, alphax, alphay, alphaz, kx, ky, kz, pc);
     for (int e = 0; e < c->cfg.nelem; ++e) c->pmlcheck[e] = pmlcheck[e] ? 1 : 0;
     int rc = upload(c, &c->pmlc, pc);
     if (rc) return rc;

@@ -41,6 +41,12 @@ constexpr int NMB = 3;
 template <typename R>
 struct StageArgs {
     int nglob;
+    int nshot;                  // shots advanced together (1 for the single-shot path)
+    int mat_stride;             // per-shot stride of matc (0: shared), in R
+    int anel_stride;            // per-shot stride of anel (0: shared)
+    int pmlc_stride_ng;         // per-shot stride of pmlc in units of nglob (0 or 9)
+    int src_factored;           // 1: sources as spatial x STF; 0: srcArray/srcArrayM
+    int nt;
     const int *elist;           // elements handled by this launch
     int nlist;
     int irk;                    // 1-based stage
@@ -76,11 +82,14 @@ struct StageArgs {
     const int *srctype;         // [nsrc]
     const R *src_array;         // [nt][nsrc][3][NP]
     const R *src_array_m;       // [nt][nsrc][6][NP]
-    int nsrc;
+    const R *src_spatial;       // [shot][nsrc][6][NP]  factored sources
+    const R *src_stf;           // [shot][nsrc][nt]
+    int nsrc;                   // sources per shot
     const int *d_it;            // current step (0-based), device-resident
     const int *rec_slot;        // [nelem] receiver-element slot or -1
-    R *u_slot;                  // [nslot][3][NP]
-    R *a_slot;                  // [nslot][3][NP]
+    int nslot;
+    R *u_slot;                  // [shot][nslot][3][NP]
+    R *a_slot;                  // [shot][nslot][3][NP]
     R *u_full;                  // [3][nglob] or null
     R *blk_energy;              // [gridDim] or null
     const R *mass;              // [NP][NP]
@@ -226,11 +235,19 @@ stage_kernel(StageArgs<R> a) {
     const int t = threadIdx.x;
     const size_t ng = (size_t)a.nglob;
     const int it_step = *a.d_it;
+    // shots vary fastest across blocks, so the blocks of one element group run
+    // together and share its geometry through L2
+    const int shot = (int)(blockIdx.x % (unsigned)a.nshot);
+    const int eblk = (int)(blockIdx.x / (unsigned)a.nshot);
+    const R *qin = a.qin + (size_t)shot * 9 * ng;
+    const R *qbase = a.qbase + (size_t)shot * 9 * ng;
+    R *qout = a.qout + (size_t)shot * 9 * ng;
+    const R *matc = a.matc + (size_t)shot * a.mat_stride;
 
     // ---- node-thread identity ------------------------------------------------
     const bool node_thr = t < NT;
     const int eb = node_thr ? t / NP : 0, j = node_thr ? t - (t / NP) * NP : 0;
-    const int idx = blockIdx.x * EB + eb;
+    const int idx = eblk * EB + eb;
     const bool valid = node_thr && idx < a.nlist;
     const int e = valid ? a.elist[idx] : 0;
     const size_t g = (size_t)e * NP + j;
@@ -240,11 +257,11 @@ stage_kernel(StageArgs<R> a) {
     if (node_thr) {
 #pragma unroll
         for (int f = 0; f < 9; ++f) {
-            qs[f] = valid ? a.qin[f * ng + g] : R(0);
+            qs[f] = valid ? qin[f * ng + g] : R(0);
             sQ[f * NT + t] = qs[f];
         }
         if (valid) {
-            const R *mc = a.matc + (size_t)e * 8;
+            const R *mc = matc + (size_t)e * 8;
             rho = mc[2];
             ef0 = -(mc[3] + 2 * mc[4]); ef1 = -mc[3]; ef2 = -mc[4]; ef3 = R(-1.0) / rho;
         }
@@ -256,11 +273,16 @@ stage_kernel(StageArgs<R> a) {
             // stage-start values, so it is done here and the auxiliaries die
             R Fm[9], Gm[9], Hm[9];
             elastic_flux_node(qs, ef0, ef1, ef2, ef3, Fm, Gm, Hm);
+            const R *pmlc = a.pmlc + (size_t)shot * a.pmlc_stride_ng * ng;
+            const R *pin = a.pin + (size_t)shot * 27 * ng;
+            const R *pbase = a.pbase + (size_t)shot * 27 * ng;
+            R *pout = a.pout + (size_t)shot * 27 * ng;
+            const R *lastf = a.last_flux + (size_t)shot * 27 * NP;
             R kk[3] = {1, 1, 1}, al[3] = {0, 0, 0}, dk[3] = {0, 0, 0};
             if (valid) {
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
-                    al[d] = a.pmlc[d * ng + g]; dk[d] = a.pmlc[(3 + d) * ng + g]; kk[d] = a.pmlc[(6 + d) * ng + g];
+                    al[d] = pmlc[d * ng + g]; dk[d] = pmlc[(3 + d) * ng + g]; kk[d] = pmlc[(6 + d) * ng + g];
                 }
             }
             int col = 0;
@@ -271,14 +293,14 @@ stage_kernel(StageArgs<R> a) {
 #pragma unroll
                 for (int i = 0; i < 9; ++i) {
                     const int c = d * 9 + i;
-                    const R cur = valid ? a.pin[c * ng + g] : R(0);
+                    const R cur = valid ? pin[c * ng + g] : R(0);
                     const R mod = (cur + FF[i]) / kk[d];
                     if (i != skip) sX[(col++) * NT + t] = mod;
-                    const R use = a.pml_compat ? a.last_flux[c * NP + j] : mod;
+                    const R use = a.pml_compat ? lastf[c * NP + j] : mod;
                     const R dadt = -al[d] * cur - dk[d] * (cur + use);
-                    const R base = (a.irk == 1 || !valid) ? cur : a.pbase[c * ng + g];
+                    const R base = (a.irk == 1 || !valid) ? cur : pbase[c * ng + g];
                     const R v = rk_combine<R>(a.scheme, a.irk, cur, base, dadt, a.dt, a.w13, a.w23);
-                    if (valid) a.pout[c * ng + g] = v;
+                    if (valid) pout[c * ng + g] = v;
                 }
             }
         }
@@ -288,7 +310,7 @@ stage_kernel(StageArgs<R> a) {
     // ---- face term: upwind flux per face node into sF ----------------------
     for (int it = t; it < EB * NFP; it += NTHR) {
         const int fb = it / NFP, k = it - fb * NFP;
-        const int fidx = blockIdx.x * EB + fb;
+        const int fidx = eblk * EB + fb;
         R *dst = sF + (fb * NFP + k) * NFCP;
         if (fidx >= a.nlist) {
 #pragma unroll
@@ -303,12 +325,12 @@ stage_kernel(StageArgs<R> a) {
         if (ft >= 0) {
             const size_t gn = (size_t)a.nbr[((size_t)fe * 4 + fi) * NPF + i];
 #pragma unroll
-            for (int f = 0; f < 9; ++f) dq[f] = a.qin[f * ng + gn] - sQ[f * NT + own];
+            for (int f = 0; f < 9; ++f) dq[f] = qin[f * ng + gn] - sQ[f * NT + own];
         } else {
 #pragma unroll
             for (int f = 0; f < 9; ++f) dq[f] = sQ[f * NT + own];
         }
-        const R *mc = a.matc + (size_t)fe * 8;
+        const R *mc = matc + (size_t)fe * 8;
         R fl[9], af[6];
         upwind_flux<R>(a.frame + ((size_t)fe * 4 + fi) * 9, mc[0], mc[1], mc[2], mc[3], mc[4], ft == -1, dq, fl, af, ATT);
         const R sgn = ft == -2 ? R(-1) : R(1);
@@ -341,7 +363,7 @@ stage_kernel(StageArgs<R> a) {
     const int ct = t;
     const bool col_thr = ct < EB * NCOL;
     const int ceb = col_thr ? ct / NCOL : 0, cc = col_thr ? ct - (ct / NCOL) * NCOL : 0;
-    const int cidx = blockIdx.x * EB + ceb;
+    const int cidx = eblk * EB + ceb;
     const bool cvalid = col_thr && cidx < a.nlist;
     const int ce = cvalid ? a.elist[cidx] : 0;
     R X[NP];
@@ -428,20 +450,23 @@ stage_kernel(StageArgs<R> a) {
         if (node_thr) {
             const R ex = gv[0], ey = gv[4], ez = gv[8];
             const R exy = gv[3] + gv[1], eyz = gv[7] + gv[5], exz = gv[6] + gv[2];
-            const R *av = a.anel + (size_t)e * NMB * 3;
+            const R *av = a.anel + (size_t)shot * a.anel_stride + (size_t)e * NMB * 3;
+            const R *thin = a.thin + (size_t)shot * 18 * ng;
+            const R *thbase = a.thbase + (size_t)shot * 18 * ng;
+            R *thout = a.thout + (size_t)shot * 18 * ng;
 #pragma unroll
             for (int i = 0; i < NMB; ++i) {
                 const R w = a.wl[i];
                 const R vol[6] = {-w * ex, -w * ey, -w * ez, R(-0.5) * w * exy, R(-0.5) * w * eyz, R(-0.5) * w * exz};
                 R th[6];
 #pragma unroll
-                for (int c = 0; c < 6; ++c) th[c] = valid ? a.thin[(i * 6 + c) * ng + g] : R(0);
+                for (int c = 0; c < 6; ++c) th[c] = valid ? thin[(i * 6 + c) * ng + g] : R(0);
 #pragma unroll
                 for (int c = 0; c < 6; ++c) {
                     const R arq = vol[c] + w * lf[9 + c] - w * th[c];
-                    const R base = (a.irk == 1 || !valid) ? th[c] : a.thbase[(i * 6 + c) * ng + g];
+                    const R base = (a.irk == 1 || !valid) ? th[c] : thbase[(i * 6 + c) * ng + g];
                     const R nv = rk_combine<R>(a.scheme, a.irk, th[c], base, arq, a.dt, a.w13, a.w23);
-                    if (valid) a.thout[(i * 6 + c) * ng + g] = nv;
+                    if (valid) thout[(i * 6 + c) * ng + g] = nv;
                 }
                 const R a1 = valid ? av[i * 3 + 0] : R(0), a2 = valid ? av[i * 3 + 1] : R(0), a3 = valid ? av[i * 3 + 2] : R(0);
                 rhs[0] = rhs[0] + a3 * th[0] + a1 * (th[1] + th[2]);
@@ -460,8 +485,21 @@ stage_kernel(StageArgs<R> a) {
         for (int f = 0; f < 9; ++f) rhs[f] = rhs[f] + lf[f];
         // sources (src/timeloopMod.f90:467-475)
         if (valid) {
-            for (int s = a.src_head[e]; s >= 0; s = a.src_next[s]) {
-                if (a.srctype[s] == 0) {
+            const int *head = a.src_head + (size_t)shot * a.nglob / NP;
+            const int *next = a.src_next + (size_t)shot * a.nsrc;
+            const int *stype = a.srctype + (size_t)shot * a.nsrc;
+            for (int s = head[e]; s >= 0; s = next[s]) {
+                if (a.src_factored) {
+                    // spatial operand times the source time function of the step
+                    const R *sp = a.src_spatial + (((size_t)shot * a.nsrc + s) * 6) * NP + j;
+                    const R amp = a.src_stf[((size_t)shot * a.nsrc + s) * a.nt + it_step];
+                    if (stype[s] == 0) {
+                        rhs[6] += sp[0] * amp; rhs[7] += sp[NP] * amp; rhs[8] += sp[2 * NP] * amp;
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < 6; ++c) rhs[c] += sp[c * NP] * amp;
+                    }
+                } else if (stype[s] == 0) {
                     const R *sp = a.src_array + (((size_t)it_step * a.nsrc + s) * 3) * NP + j;
                     rhs[6] += sp[0]; rhs[7] += sp[NP]; rhs[8] += sp[2 * NP];
                 } else {
@@ -474,9 +512,9 @@ stage_kernel(StageArgs<R> a) {
         // Runge-Kutta update
 #pragma unroll
         for (int f = 0; f < 9; ++f) {
-            const R base = (a.irk == 1 || !valid) ? qs[f] : a.qbase[f * ng + g];
+            const R base = (a.irk == 1 || !valid) ? qs[f] : qbase[f * ng + g];
             qnew[f] = rk_combine<R>(a.scheme, a.irk, qs[f], base, rhs[f], a.dt, a.w13, a.w23);
-            if (valid) a.qout[f * ng + g] = qnew[f];
+            if (valid) qout[f * ng + g] = qnew[f];
         }
     }
 
@@ -489,11 +527,12 @@ stage_kernel(StageArgs<R> a) {
             }
             const int slot = a.rec_slot[e];
             if (slot >= 0) {
+                const size_t so = (size_t)shot * a.nslot + slot;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    R *u = a.u_slot + ((size_t)slot * 3 + c) * NP + j;
+                    R *u = a.u_slot + (so * 3 + c) * NP + j;
                     *u = *u + a.dt * qnew[6 + c];
-                    a.a_slot[((size_t)slot * 3 + c) * NP + j] = rhs[6 + c];
+                    a.a_slot[(so * 3 + c) * NP + j] = rhs[6 + c];
                 }
             }
         }
@@ -548,15 +587,21 @@ stage_kernel(StageArgs<R> a) {
     }
 }
 
-// Fluxes of the last element for the PML ADE in compat mode (H1): one
-// thread per node of element nelem-1.
+// Fluxes of the last element for the PML ADE in compat mode (H1): one block
+// per shot, one thread per node of element nelem-1.
 template <int ORDER, typename R>
 __global__ void last_flux_kernel(const R *qin, const R *pin, const R *pmlc, const R *matc, int last, int last_is_pml,
-                                 int nglob, R *out) {
+                                 int nglob, int mat_stride, int pmlc_stride_ng, R *out) {
     constexpr int NP = Dims<ORDER>::NP;
     const int j = threadIdx.x;
+    const int shot = blockIdx.x;
     if (j >= NP) return;
     const size_t ng = (size_t)nglob, g = (size_t)last * NP + j;
+    qin += (size_t)shot * 9 * ng;
+    pin += (size_t)shot * 27 * ng;
+    pmlc += (size_t)shot * pmlc_stride_ng * ng;
+    matc += (size_t)shot * mat_stride;
+    out += (size_t)shot * 27 * NP;
     R q[9];
 #pragma unroll
     for (int f = 0; f < 9; ++f) q[f] = qin[f * ng + g];
@@ -581,14 +626,14 @@ __global__ void last_flux_kernel(const R *qin, const R *pin, const R *pmlc, cons
     }
 }
 
-// Receiver sampling (src/timeloopMod.f90:358-373) of row `row_offset + *d_it`,
-// the energy of the step (sum of the stage kernels' block partials in a fixed
-// order), and the step-counter advance.  One block of 1024 threads.
+// Receiver sampling (src/timeloopMod.f90:358-373) of row `row_offset + *d_it`
+// for every shot, the energy of the step (sum of the stage kernels' block
+// partials in a fixed order), and the step-counter advance.  One block.
 template <int ORDER, typename R>
 __global__ void __launch_bounds__(1024)
 step_end_kernel(const R *q, const R *u_slot, const R *a_slot, const R *rec_int, const int *recelem,
-                const int *rec_slot_of, int nrec, int nglob, int nt, int row_offset, int advance,
-                R *seis, const R *blk_energy, int nblk, R *energy_out, int *d_it) {
+                const int *rec_slot_of, int nrec, int nslot, int nshot, int nglob, int nt, int row_offset,
+                int advance, R *seis, const R *blk_energy, int nblk, R *energy_out, int *d_it) {
     constexpr int NP = Dims<ORDER>::NP;
     __shared__ int s_it;
     __shared__ R s_red[1024];
@@ -597,19 +642,23 @@ step_end_kernel(const R *q, const R *u_slot, const R *a_slot, const R *rec_int, 
     const int it = s_it;
     const int row = it + row_offset;
     if (row < nt) {
-        for (int w = threadIdx.x; w < nrec * 9; w += blockDim.x) {
-            const int r = w / 9, k = w - r * 9;
-            const int e = recelem[r], slot = rec_slot_of[r];
+        for (int w = threadIdx.x; w < nshot * nrec * 9; w += blockDim.x) {
+            const int shot = w / (nrec * 9);
+            const int rk = w - shot * nrec * 9;
+            const int r = rk / 9, k = rk - r * 9;
+            const int e = recelem[r];
+            const size_t so = (size_t)shot * nslot + rec_slot_of[r];
             const R *wt = rec_int + (size_t)r * NP;
+            const R *qs = q + (size_t)shot * 9 * nglob;
             R acc = 0;
             for (int jn = 0; jn < NP; ++jn) {
                 R v;
-                if (k < 3) v = u_slot[((size_t)slot * 3 + k) * NP + jn];
-                else if (k < 6) v = q[(size_t)(6 + k - 3) * nglob + (size_t)e * NP + jn];
-                else v = a_slot[((size_t)slot * 3 + (k - 6)) * NP + jn];
+                if (k < 3) v = u_slot[(so * 3 + k) * NP + jn];
+                else if (k < 6) v = qs[(size_t)(6 + k - 3) * nglob + (size_t)e * NP + jn];
+                else v = a_slot[(so * 3 + (k - 6)) * NP + jn];
                 acc = acc + v * wt[jn];
             }
-            seis[((size_t)k * nt + row) * nrec + r] = acc;
+            seis[(((size_t)shot * 9 + k) * nt + row) * nrec + r] = acc;
         }
     }
     if (advance && blk_energy) {

@@ -173,3 +173,126 @@ class Context:
 
     def __exit__(self, *exc):
         self.close()
+
+
+# ---------------------------------------------------------------------------
+# batched shots (nexd_batch_*)
+# ---------------------------------------------------------------------------
+NEXD_FP64, NEXD_FP32 = 0, 1
+
+BATCH_EXPORTS = ["nexd_batch_create", "nexd_batch_upload_operators", "nexd_batch_upload_mesh",
+                 "nexd_batch_upload_material", "nexd_batch_upload_anelastic", "nexd_batch_upload_pml",
+                 "nexd_batch_upload_sources", "nexd_batch_upload_receivers", "nexd_batch_reset", "nexd_batch_run",
+                 "nexd_batch_get_seismograms", "nexd_batch_last_run_ms", "nexd_batch_time_stage_kernel",
+                 "nexd_batch_destroy", "nexd_batch_last_error"]
+
+
+class NexdBatchConfig(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("nelem", "order", "nshot", "nsrc", "nrec", "nt", "timeint",
+                                            "attenuation", "pml", "compat", "precision", "shared_material",
+                                            "device")] + [("dt", ctypes.c_double)]
+
+
+def _load_batch():
+    lib = load()
+    if not getattr(lib, "_batch_ready", False):
+        lib.nexd_batch_create.argtypes = [ctypes.POINTER(NexdBatchConfig), ctypes.POINTER(ctypes.c_void_p)]
+        lib.nexd_batch_last_error.restype = ctypes.c_char_p
+        lib.nexd_batch_last_error.argtypes = [ctypes.c_void_p]
+        lib.nexd_batch_destroy.restype = None
+        lib.nexd_batch_destroy.argtypes = [ctypes.c_void_p]
+        for n in BATCH_EXPORTS:
+            if n not in ("nexd_batch_last_error", "nexd_batch_destroy"):
+                getattr(lib, n).restype = ctypes.c_int
+        lib._batch_ready = True
+    return lib
+
+
+class BatchContext:
+    """Many shots on one mesh (nexd_amd.batch.ShotBatch), FP32 or FP64."""
+
+    def __init__(self, sb, precision: int = NEXD_FP32, compat: int = COMPAT_ALL, device: int = 0):
+        from . import layout
+        self.lib = _load_batch()
+        base = sb.base
+        a = layout.reference_arrays(base)
+        self.nshot, self.nt, self.nrec = sb.nshot, base.nt, a["nrec"]
+        cfg = NexdBatchConfig(nelem=a["nelem"], order=a["order"], nshot=sb.nshot, nsrc=len(sb.shots[0].sources),
+                              nrec=a["nrec"], nt=a["nt"], timeint=a["timeint"], attenuation=a["attenuation"],
+                              pml=a["pml"], compat=compat, precision=precision,
+                              shared_material=int(sb.shared_material), device=device, dt=base.dt)
+        h = ctypes.c_void_p()
+        rc = self.lib.nexd_batch_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != 0:
+            raise NexdError(f"nexd_batch_create: {rc}: {self.lib.nexd_batch_last_error(None).decode()}")
+        self.h = h
+        L = self.lib
+        self._chk(L.nexd_batch_upload_operators(h, _d(a["dr"]), _d(a["ds"]), _d(a["dt_"]), _d(a["lift"]), _d(a["mass"]),
+                                                _i(a["fmask"])), "upload_operators")
+        self._chk(L.nexd_batch_upload_mesh(h, _i(a["ibool"]), *[_d(a[k]) for k in (
+            "rx", "ry", "rz", "sx", "sy", "sz", "tx", "ty", "tz", "jac", "fscale")], _i(a["neighbor"]), _i(a["face"]),
+            _i(a["ibt"]), _i(a["ibn"]), _d(a["nnormal"]), _d(a["stangential"]), _d(a["ttangential"])), "upload_mesh")
+        nsets = 1 if sb.shared_material else sb.nshot
+        for k in range(nsets):
+            ak = layout.reference_arrays(sb.shots[k]) if k else layout.reference_arrays(sb.shots[0])
+            self._chk(L.nexd_batch_upload_material(h, k, *[_d(ak[x]) for x in ("vpu", "vsu", "rho", "lambdau", "muu")]),
+                      "upload_material")
+            if ak["attenuation"]:
+                self._chk(L.nexd_batch_upload_anelastic(h, k, _d(ak["wl"]), _d(ak["anelasticvar"])), "upload_anelastic")
+            if ak["pml"]:
+                self._chk(L.nexd_batch_upload_pml(h, k, _i(ak["pmlcheck"]), *[_d(ak[x]) for x in (
+                    "ddx", "ddy", "ddz", "alphax", "alphay", "alphaz", "kx", "ky", "kz")]), "upload_pml")
+        for k, s in enumerate(sb.shots):
+            so = s.src
+            srcelem = np.ascontiguousarray(so.srcelem + 1, dtype=np.int32)
+            srctype = np.ascontiguousarray(so.srctype, dtype=np.int32)
+            spatial = np.ascontiguousarray(so.spatial, dtype=np.float64)
+            stf = np.ascontiguousarray(so.stf, dtype=np.float64)
+            self._chk(L.nexd_batch_upload_sources(h, k, _i(srcelem), _i(srctype), _d(spatial), _d(stf)),
+                      "upload_sources")
+        if a["nrec"]:
+            self._chk(L.nexd_batch_upload_receivers(h, _i(a["recelem"]), _d(a["rec_int"])), "upload_receivers")
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise NexdError(f"{what}: {rc}: {self.lib.nexd_batch_last_error(self.h).decode()}")
+
+    def reset(self):
+        self._chk(self.lib.nexd_batch_reset(self.h), "reset")
+
+    def run(self, it_begin: int, it_end: int):
+        self._chk(self.lib.nexd_batch_run(self.h, it_begin, it_end), "run")
+
+    def seismograms(self, shot: int) -> np.ndarray:
+        outs = [np.zeros(self.nrec * self.nt) for _ in range(9)]
+        self._chk(self.lib.nexd_batch_get_seismograms(self.h, shot, *[_d(o) for o in outs]), "get_seismograms")
+        return np.stack([o.reshape(self.nt, self.nrec).T for o in outs], axis=1)
+
+    def last_run_ms(self) -> float:
+        ms = ctypes.c_double()
+        self._chk(self.lib.nexd_batch_last_run_ms(self.h, ctypes.byref(ms)), "last_run_ms")
+        return ms.value
+
+    def time_stage_kernel(self, lst: int, nrep: int = 20):
+        ms = ctypes.c_double()
+        n = ctypes.c_int()
+        self._chk(self.lib.nexd_batch_time_stage_kernel(self.h, lst, nrep, ctypes.byref(ms), ctypes.byref(n)),
+                  "time_stage_kernel")
+        return ms.value, n.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nexd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -173,3 +173,30 @@ def from_reference_example(cubit_dir: str, order: int, par: dict, sources, recei
     """A set-up from the reference's ``cubit/`` mesher inputs."""
     m = meshmod.read_cubit(cubit_dir)
     return setup(m, order, par, sources, receivers, name=name)
+
+
+def from_mesh_arrays(coord, elem, props, mat, free, absorb, matprop, order: int, par: dict, sources, receivers,
+                     name: str = "") -> Problem:
+    """A set-up from the numbers of the reference's cubit/ files: elem,
+    free and absorb rows are 1-based as in the files; props = vp vs rho qp qs
+    per element; mat = (id, material, pml) rows."""
+    m = meshmod.TetMesh(coord=np.asarray(coord, float), elem=np.asarray(elem, np.int64) - 1,
+                        vp=props[:, 0].copy(), vs=props[:, 1].copy(), rho=props[:, 2].copy(),
+                        qp=props[:, 3].copy(), qs=props[:, 4].copy(),
+                        mat=np.asarray(mat)[:, 1].astype(np.int64), pml=np.asarray(mat)[:, 2].astype(np.int64),
+                        free=np.asarray(free, np.int64) - 1, absorb=np.asarray(absorb, np.int64) - 1,
+                        matprop=np.asarray(matprop, float))
+    return setup(m, order, par, sources, receivers, name=name)
+
+
+def brick_example(path: str, nt: int | None = None) -> Problem:
+    """The reference's simulations/brick example (10 m cube, 9,859 tets, N=4,
+    z-force Ricker at 500 Hz, 11 receivers) from the packed fixture."""
+    d = np.load(path)
+    timeint, nt0, cfl, _ = d["par"]
+    sx, sy, sz, stype, stf, f0, factor, ax, ay, az = d["source"]
+    src = srcmod.Source(xyz=(sx, sy, sz), kind=int(stype), stf=int(stf), f0=float(f0), factor=float(factor),
+                        angle_force=(ax, ay, az))
+    return from_mesh_arrays(d["coord"], d["elem"], d["props"], d["mat"], d["free"], d["absorb"], d["matprop"], 4,
+                            dict(timeint=int(timeint), nt=int(nt or nt0), cfl=float(cfl)), [src], d["stations"],
+                            name="brick")
