@@ -77,72 +77,126 @@ def cpu_baseline(order=4):
                       f"N=4 anelastic NPML box, {nt} RK2 steps, loop time {loop_s:.2f} s (set-up excluded)"}
 
 
+def rel_l2_max(a, b):
+    import numpy as np
+    num = np.linalg.norm(a - b, axis=-1)
+    den = np.maximum(np.linalg.norm(b, axis=-1), 1e-300)
+    return float(np.max(num / den))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--shots", type=int, default=8, help="shots in the FP32 batched run")
     ap.add_argument("--nx", type=int, default=40)
     ap.add_argument("--ny", type=int, default=40)
     ap.add_argument("--nz", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe * (1 << 20) // 4
+        src = torch.arange(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 5}))
+        return
 
-    from nexd_amd import capi, layout, problem
+    from nexd_amd import batch, capi, layout
 
     nt = args.warmup + args.steps + 1
     t0 = time.time()
-    p = problem.layered_box(nx=args.nx, ny=args.ny, nz=args.nz, order=4, nt=nt, pml_cells=10 if args.nx >= 40 else 2,
-                            nrec_side=8, timeint=2, attenuation=True)
-    arrs = layout.reference_arrays(p)
-    log(f"[bench] set-up {time.time() - t0:.1f}s: nelem={p.nelem} N={p.order} PML elements={int(p.mesh.pml.sum())}")
-    ctx = capi.Context(arrs, compat=capi.COMPAT_ALL, energy=True)
-    ctx.reset()
-    if args.warmup:
-        ctx.run(1, args.warmup)
-    log(f"[bench] warm-up {args.warmup} steps done")
-    t1 = time.perf_counter()
-    ctx.run(args.warmup + 1, args.warmup + args.steps)
-    wall = time.perf_counter() - t1
-    dev_ms = ctx.last_run_ms()
-    log(f"[bench] timed {args.steps} steps: wall {wall:.3f}s device {dev_ms:.1f} ms")
+    pml_cells = 10 if args.nx >= 40 else 2
+    sb = batch.layered_shots(args.shots, nx=args.nx, ny=args.ny, nz=args.nz, order=4, nt=nt, pml_cells=pml_cells,
+                             nrec_side=8, timeint=2, attenuation=True, full_source_arrays=True)
+    p = sb.shots[0]
+    log(f"[bench] set-up {time.time() - t0:.1f}s: nelem={p.nelem} N={p.order} PML elements={int(p.mesh.pml.sum())} "
+        f"shots={sb.nshot}")
     dof_step = p.nelem * p.ref.np_ * 9
-    value = dof_step * args.steps / wall
+    nsteps_all = args.warmup + args.steps
 
-    # dominant kernel: RK stage 1 over the PML elements
-    ms_pml, n_pml = ctx.time_stage_kernel(1, nrep=20)
-    ms_plain, n_plain = ctx.time_stage_kernel(0, nrep=20)
-    b_pml = algorithmic_bytes_pml_att_stage1(p.ref.np_, p.ref.npf) * n_pml
-    achieved = b_pml / (ms_pml * 1e-3) / 1e9
-    log(f"[bench] PML stage kernel {ms_pml:.3f} ms for {n_pml} elements -> {achieved:.0f} GB/s; "
-        f"interior stage kernel {ms_plain:.3f} ms for {n_plain} elements")
-    ctx.close()
+    # ---- FP64 plain path (the drop-in), shot 0 ------------------------------
+    fp64 = None
+    if not args.skip_fp64:
+        arrs = layout.reference_arrays(p)
+        ctx = capi.Context(arrs, compat=capi.COMPAT_ALL, energy=True)
+        ctx.reset()
+        if args.warmup:
+            ctx.run(1, args.warmup)
+        t1 = time.perf_counter()
+        ctx.run(args.warmup + 1, nsteps_all)
+        wall64 = time.perf_counter() - t1
+        seis64 = ctx.seismograms()[:, :, :nsteps_all]
+        ms_pml, n_pml = ctx.time_stage_kernel(1, nrep=20)
+        ms_plain, n_plain = ctx.time_stage_kernel(0, nrep=20)
+        ctx.close()
+        b_pml = algorithmic_bytes_pml_att_stage1(p.ref.np_, p.ref.npf) * n_pml
+        fp64 = dict(value=dof_step * args.steps / wall64, ms_per_step=wall64 / args.steps * 1e3,
+                    ms_pml=ms_pml, n_pml=n_pml, ms_plain=ms_plain, n_plain=n_plain,
+                    achieved=b_pml / (ms_pml * 1e-3) / 1e9)
+        log(f"[bench] FP64 plain: {fp64['ms_per_step']:.2f} ms/step, PML stage kernel {ms_pml:.3f} ms "
+            f"({fp64['achieved']:.0f} GB/s), interior {ms_plain:.3f} ms")
+
+    # ---- FP32 batched path, all shots ----------------------------------------
+    bctx = capi.BatchContext(sb, precision=capi.NEXD_FP32, compat=capi.COMPAT_ALL)
+    bctx.reset()
+    if args.warmup:
+        bctx.run(1, args.warmup)
+    t1 = time.perf_counter()
+    bctx.run(args.warmup + 1, nsteps_all)
+    wall32 = time.perf_counter() - t1
+    seis32 = bctx.seismograms(0)[:, :, :nsteps_all]
+    ms32_pml, n32_pml = bctx.time_stage_kernel(1, nrep=10)
+    bctx.close()
+    value32 = dof_step * sb.nshot * args.steps / wall32
+    b32 = algorithmic_bytes_pml_att_stage1(p.ref.np_, p.ref.npf, w=4) * n32_pml * sb.nshot
+    achieved32 = b32 / (ms32_pml * 1e-3) / 1e9
+    err32 = rel_l2_max(seis32, seis64) if fp64 else None
+    log(f"[bench] FP32 batched ({sb.nshot} shots): {wall32 / args.steps * 1e3:.2f} ms/step, PML stage kernel "
+        f"{ms32_pml:.3f} ms ({achieved32:.0f} GB/s), seismogram error vs FP64 over {nsteps_all} steps: {err32}")
 
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     out = {
         "metric": "DOF-updates/s on 1 GPU: FP64 plain vs FP32 batched at ≤1e-4 seismogram L2",
-        "value": value,
+        "value": value32,
         "unit": "DOF-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": wall32 / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"two-layer anelastic NPML box {args.nx}x{args.ny}x{args.nz} hexes "
-                               f"({p.nelem} tets), N=4, 3 SLS, RK2, FP64 plain path",
-                   "dof_per_step": dof_step, "rk_stages": 2, "path": "fp64_plain"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "stage_kernel<4,ATT,PML,f64> (RK stage 1, PML elements)",
-                     "bytes_per_element": algorithmic_bytes_pml_att_stage1(p.ref.np_, p.ref.npf),
-                     "avg_launch_ms": ms_pml, "elements_per_launch": n_pml},
+                               f"({p.nelem} tets), N=4, 3 SLS, RK2; FP32 batched path with {sb.nshot} shots "
+                               f"(one moment-tensor source each) in one launch per stage",
+                   "dof_per_step": dof_step * sb.nshot, "rk_stages": 2, "shots": sb.nshot,
+                   "fp64_plain_dof_updates_per_s": fp64["value"] if fp64 else None,
+                   "fp64_plain_ms_per_step": fp64["ms_per_step"] if fp64 else None,
+                   "fp32_over_fp64_shot_throughput": (value32 / fp64["value"]) if fp64 else None,
+                   "fp32_seismogram_rel_l2_vs_fp64": err32,
+                   "accuracy_steps": nsteps_all},
+        "roofline": {"bound": "hbm", "achieved": achieved32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved32 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "stage_kernel<4,ATT,PML,f32> (RK stage 1, PML elements, all shots)",
+                     "bytes_per_element": algorithmic_bytes_pml_att_stage1(p.ref.np_, p.ref.npf, w=4),
+                     "avg_launch_ms": ms32_pml, "elements_per_launch": n32_pml * sb.nshot,
+                     "fp64_kernel": ({"achieved": fp64["achieved"], "frac": fp64["achieved"] / HBM_PEAK_GBS,
+                                      "avg_launch_ms": fp64["ms_pml"], "elements_per_launch": fp64["n_pml"],
+                                      "bytes_per_element": algorithmic_bytes_pml_att_stage1(p.ref.np_, p.ref.npf)}
+                                     if fp64 else None)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

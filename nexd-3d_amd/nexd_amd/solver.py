@@ -1,0 +1,118 @@
+"""Host driver for a reference-format simulation directory: the role of the
+reference's mesher + ``program_solver`` for one GPU, with the time loop in
+libnexd_gpu.so.
+
+A directory holds ``cubit/{coord,mesh,mat,matprop,free,absorb}`` and
+``data/{parfile,source,stations}`` (src/meshMod.f90:146-319,
+src/parameterMod.f90:30-103, src/sourceReceiverMod.f90:36-201).  ``run``
+builds the operands, advances all steps on the GPU and writes
+``out/seismo.{x,y,z}.%07d.sd{u,v,a}`` as the reference does
+(src/timeloopMod.f90:591-619, src/plotMod.f90:27-38).
+
+Parameter values are read from at most the first 10 characters after '=',
+and lines from their first 50 characters, as the reference reads them
+(src/fileParameterMod.f90:65,97-129,150).
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from . import capi, layout, mesh as meshmod, problem, sources as srcmod
+
+
+def _raw_pairs(path: str):
+    pairs = []
+    with open(path) as f:
+        for ln in f:
+            ln = ln[:50]
+            if ln.lstrip().startswith("#") or "=" not in ln:
+                continue
+            k, v = ln.split("=", 1)
+            pairs.append((k.strip(), v.strip()[:10].split("#")[0].split()[0] if v.strip() else ""))
+    return pairs
+
+
+def _value(v: str):
+    low = v.lower()
+    if low.startswith(".true") or low == "t":
+        return True
+    if low.startswith(".false") or low == "f":
+        return False
+    try:
+        return int(v)
+    except ValueError:
+        return float(v.replace("d", "e").replace("D", "e"))
+
+
+def read_parfile(path: str) -> dict:
+    par = dict(problem.DEFAULT_PAR)
+    for k, v in _raw_pairs(path):
+        if k in par:
+            par[k] = _value(v)
+    return par
+
+
+def read_sources(path: str):
+    pairs = _raw_pairs(path)
+    nsrc = next(int(_value(v)) for k, v in pairs if k == "nsrc")
+    blocks, cur = [], None
+    for k, v in pairs:
+        if k == "source":
+            cur = {}
+            blocks.append(cur)
+        elif cur is not None:
+            cur[k] = _value(v)
+    out = []
+    for b in blocks[:nsrc]:
+        out.append(srcmod.Source(xyz=(b["xsource"], b["ysource"], b["zsource"]), kind=int(b["sourcetype"]),
+                                 stf=int(b["stf"]), f0=float(b["f0"]), factor=float(b["factor"]),
+                                 angle_force=(b["xangle_force"], b["yangle_force"], b["zangle_force"]),
+                                 M=(b["Mxx"], b["Myy"], b["Mzz"], b["Mxy"], b["Mxz"], b["Myz"])))
+    return out
+
+
+def read_stations(path: str):
+    with open(path) as f:
+        lines = [ln.split() for ln in f if ln.strip()]
+    n = int(lines[0][0])
+    nr = [int(r[0]) for r in lines[1:n + 1]]
+    xyz = np.array([[float(v) for v in r[1:4]] for r in lines[1:n + 1]])
+    return nr, xyz
+
+
+def write_seismograms(outdir: str, seis: np.ndarray, dt: float, recnr=None) -> None:
+    """seis (nrec, 9, nt): ux uy uz vx vy vz ax ay az; time column it*dt."""
+    os.makedirs(outdir, exist_ok=True)
+    nrec, _, nt = seis.shape
+    t = np.arange(1, nt + 1) * dt
+    names = [(c, q) for q in ("sdu", "sdv", "sda") for c in "xyz"]
+    for r in range(nrec):
+        nr = recnr[r] if recnr is not None else r + 1
+        for k, (c, q) in enumerate(names):
+            with open(os.path.join(outdir, f"seismo.{c}.{nr:07d}.{q}"), "w") as f:
+                for i in range(nt):
+                    f.write(f" {t[i]:.16E} {seis[r, k, i]:.16E}\n")
+
+
+def setup_directory(dirname: str, order: int) -> problem.Problem:
+    m = meshmod.read_cubit(os.path.join(dirname, "cubit"))
+    par = read_parfile(os.path.join(dirname, "data", "parfile"))
+    srcs = read_sources(os.path.join(dirname, "data", "source"))
+    nr, xyz = read_stations(os.path.join(dirname, "data", "stations"))
+    p = problem.setup(m, order, par, srcs, xyz, name=os.path.basename(os.path.abspath(dirname)))
+    p.extra["recnr"] = nr
+    return p
+
+
+def run(dirname: str, order: int, device: int = 0, compat: int = capi.COMPAT_ALL) -> np.ndarray:
+    """Advance the whole run on the GPU and write out/seismo.* files."""
+    p = setup_directory(dirname, order)
+    arrs = layout.reference_arrays(p)
+    with capi.Context(arrs, compat=compat, device=device, energy=bool(p.set_pml)) as ctx:
+        ctx.reset()
+        ctx.run(1, p.nt)
+        seis = ctx.seismograms()
+    write_seismograms(os.path.join(dirname, "out"), seis, p.dt, p.extra.get("recnr"))
+    return seis

@@ -1,0 +1,39 @@
+"""Reference file formats: simulation-directory inputs (cubit/, data/) and
+the seismogram output, read back through the host driver."""
+import os
+
+import numpy as np
+
+from nexd_amd import database, mesh as meshmod, problem, solver
+
+
+def test_directory_round_trip(tmp_path):
+    p0 = problem.homogeneous_cube(n=2, order=2, nt=20, source_kind=1)
+    d = str(tmp_path)
+    meshmod.write_cubit(os.path.join(d, "cubit"), p0.mesh)
+    os.makedirs(os.path.join(d, "data"))
+    database.write_parfile(os.path.join(d, "data", "parfile"), p0.par)
+    database.write_source_file(os.path.join(d, "data", "source"), p0.sources)
+    database.write_stations(os.path.join(d, "data", "stations"), p0.rec_xyz)
+    p1 = solver.setup_directory(d, 2)
+    assert p1.nelem == p0.nelem and p1.nt == p0.nt and p1.timeint == p0.timeint
+    assert abs(p1.dt - p0.dt) <= 1e-15 * p0.dt
+    assert np.array_equal(p1.src.srcelem, p0.src.srcelem)
+    assert np.allclose(p1.src.spatial, p0.src.spatial, rtol=1e-12, atol=0)
+    assert np.allclose(p1.rec_int, p0.rec_int, rtol=0, atol=1e-14)
+
+
+def test_parameter_values_truncated_to_ten_characters(tmp_path):
+    fn = tmp_path / "parfile"
+    fn.write_text("nt = 123456789012345\ncfl = 0.12345678901234\n")
+    par = solver.read_parfile(str(fn))
+    assert par["nt"] == 1234567890
+    assert par["cfl"] == 0.12345678
+
+
+def test_seismogram_files_round_trip(tmp_path):
+    rng = np.random.default_rng(0)
+    seis = rng.standard_normal((3, 9, 17))
+    solver.write_seismograms(str(tmp_path), seis, 0.01, recnr=[4, 5, 6])
+    back = database.read_seismograms(str(tmp_path), 3, 17, recnr=[4, 5, 6])
+    assert np.array_equal(back, seis)
