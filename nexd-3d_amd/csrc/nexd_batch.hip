@@ -203,7 +203,7 @@ struct Batch : BatchBase {
 
     template <int ORDER, bool ATT, bool PML>
     void launch_one(const nexd::StageArgs<R> &a) {
-        using Sh = nexd::Shape<ORDER, ATT, PML>;
+        using Sh = nexd::Shape<ORDER, ATT, PML, R>;
         const unsigned grid = (unsigned)((a.nlist + Sh::EB - 1) / Sh::EB) * (unsigned)a.nshot;
         if (grid == 0) return;
         hipLaunchKernelGGL((nexd::stage_kernel<ORDER, ATT, PML, R>), dim3(grid), dim3(Sh::NTHR), 0, stream, a);

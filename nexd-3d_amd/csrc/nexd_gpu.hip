@@ -122,7 +122,7 @@ void drop_graph(nexd_ctx *c) {
 // ---------------------------------------------------------------------------
 template <int ORDER, bool ATT, bool PML>
 void launch_one(const StageArgs<double> &a, hipStream_t s) {
-    using S = nexd::Shape<ORDER, ATT, PML>;
+    using S = nexd::Shape<ORDER, ATT, PML, double>;
     const int grid = (a.nlist + S::EB - 1) / S::EB * a.nshot;
     if (grid == 0) return;
     hipLaunchKernelGGL((nexd::stage_kernel<ORDER, ATT, PML, double>), dim3(grid), dim3(S::NTHR), 0, s, a);
@@ -139,10 +139,10 @@ void launch_stage(const StageArgs<double> &a, bool att, bool pml, hipStream_t s)
 template <int ORDER>
 int blocks_for(int n, bool att, bool pml) {
     int eb;
-    if (att && pml) eb = nexd::Shape<ORDER, true, true>::EB;
-    else if (att) eb = nexd::Shape<ORDER, true, false>::EB;
-    else if (pml) eb = nexd::Shape<ORDER, false, true>::EB;
-    else eb = nexd::Shape<ORDER, false, false>::EB;
+    if (att && pml) eb = nexd::Shape<ORDER, true, true, double>::EB;
+    else if (att) eb = nexd::Shape<ORDER, true, false, double>::EB;
+    else if (pml) eb = nexd::Shape<ORDER, false, true, double>::EB;
+    else eb = nexd::Shape<ORDER, false, false, double>::EB;
     return (n + eb - 1) / eb;
 }
 

@@ -185,43 +185,63 @@ __device__ constexpr int kPmlCols = 24;
 // the update; column threads (element-in-block, column) for the volume
 // term, where one column of q (or of the PML-modified fluxes) stays in
 // registers while the rows of Dr/Ds/Dt stream through scalar loads.
-template <int ORDER, bool ATT, bool PML>
+template <int ORDER, bool ATT, bool PML, typename R>
 struct Shape {
     static constexpr int NP = Dims<ORDER>::NP, NPF = Dims<ORDER>::NPF, NFP = Dims<ORDER>::NFP;
     static constexpr int NCOL = PML ? (ATT ? 27 : 24) : 9;     // columns per element
     static constexpr int NV = PML ? (ATT ? 33 : 24) : 27;      // volume values per node
     static constexpr int NFC = ATT ? 15 : 9;                    // face values per face node
     static constexpr int NFCP = ATT ? 16 : 10;                  // padded for 16-byte LDS reads
-    static constexpr int lds_doubles(int eb) {
+    // four waves per block: the node phases use EB*NP of them, the column
+    // phase splits its rows over as many wave groups as fit (NRG below)
+    static constexpr int NTHR = 256;
+    // blocks per compute unit the shape is sized for (160 KiB of LDS per CU):
+    // two FP64 blocks or four FP32 blocks, i.e. 2 or 4 waves per SIMD
+    static constexpr int BPC = sizeof(R) == 8 ? 2 : 4;
+    static constexpr int LDS_BYTES = 160 * 1024 / BPC;
+    // LDS regions, in reals: q and PML flux columns (a) and face values (b)
+    // until the columns are in registers, then volume values (c) over both;
+    // the metric (m) lives through the volume term; the energy values and the
+    // block reduction alias everything at the end
+    static constexpr int off_metric(int eb) {
         const int nt = eb * NP;
-        const int a = (9 + (PML ? 24 : 0)) * nt;   // q and PML flux columns
-        const int b = eb * NFP * NFCP;             // face values
-        const int c = nt * NV;                     // volume values (aliases a and b)
-        const int e = 16 * nt;                     // energy (aliases everything)
-        const int ab = a + b;
-        const int m = ab > c ? ab : c;
+        const int a = (9 + (PML ? 24 : 0)) * nt;
+        const int b = eb * NFP * NFCP;
+        const int c = nt * NV;
+        return a + b > c ? a + b : c;
+    }
+    static constexpr int lds_reals(int eb) {
+        const int m = off_metric(eb) + 9 * eb * NP;
+        const int e = 15 * eb * NP + NTHR;
         return m > e ? m : e;
     }
     static constexpr int pick_eb() {
-        int eb = (256 / NP) > 0 ? (256 / NP) : 1;
-        while (eb > 1 && lds_doubles(eb) * 8 > 76 * 1024) --eb;
+        int eb = 256 / NP < 256 / NCOL ? 256 / NP : 256 / NCOL;
+        if (eb < 1) eb = 1;
+        while (eb > 1 && lds_reals(eb) * (int)sizeof(R) > LDS_BYTES) --eb;
         return eb;
     }
     static constexpr int EB = pick_eb();
     static constexpr int NT = EB * NP;
     static constexpr int NCT = EB * NCOL;
-    static constexpr int NTHR = ((NT > NCT ? NT : NCT) + 63) / 64 * 64 > 256 ? 256
-                                : ((NT > NCT ? NT : NCT) + 63) / 64 * 64;
-    static constexpr int LDS = lds_doubles(EB);
+    static constexpr int LDS = lds_reals(EB);
+    static constexpr int OFF_M = off_metric(EB);
+    // column threads: CT columns padded to whole waves (CTP); the rows of
+    // the volume product are split into NRG groups, one group per CTP
+    // threads, so every wave works on one row range (uniform operator loads)
+    static constexpr int CT = NCT;
+    static constexpr int CTP = (CT + 63) / 64 * 64;
+    static constexpr int NRG = (NTHR / CTP) > 0 ? NTHR / CTP : 1;
+    static constexpr int RPG = (NP + NRG - 1) / NRG;
 };
 
 template <typename R>
 using cptr = const __attribute__((address_space(4))) R *;   // uniform, read-only: scalar loads
 
 template <int ORDER, bool ATT, bool PML, typename R>
-__global__ void __launch_bounds__((Shape<ORDER, ATT, PML>::NTHR))
+__global__ void __launch_bounds__((Shape<ORDER, ATT, PML, R>::NTHR), (Shape<ORDER, ATT, PML, R>::BPC))
 stage_kernel(StageArgs<R> a) {
-    using S = Shape<ORDER, ATT, PML>;
+    using S = Shape<ORDER, ATT, PML, R>;
     constexpr int NP = S::NP, NPF = S::NPF, NFP = S::NFP, EB = S::EB, NT = S::NT;
     constexpr int NCOL = S::NCOL, NV = S::NV, NFC = S::NFC, NFCP = S::NFCP, NTHR = S::NTHR;
     __shared__ __align__(16) R lds[S::LDS];
@@ -229,8 +249,10 @@ stage_kernel(StageArgs<R> a) {
     R *sX = lds + 9 * NT;                 // [24][NT]    PML flux columns
     R *sF = lds + (9 + (PML ? 24 : 0)) * NT;   // [EB][NFP][NFCP]
     R *sV = lds;                          // [NT][NV]    after the columns are in registers
+    R *sM = lds + S::OFF_M;               // [9][NT]     metric of the block's nodes
     constexpr int NRED = NTHR <= 64 ? 64 : (NTHR <= 128 ? 128 : 256);   // power of two >= NTHR
-    __shared__ R sRed[NRED];
+    static_assert(NRED == NTHR, "the block reduction assumes a power-of-two block");
+    R *sRed = lds + 15 * NT;              // [NTHR]      energy reduction
 
     const int t = threadIdx.x;
     const size_t ng = (size_t)a.nglob;
@@ -260,6 +282,8 @@ stage_kernel(StageArgs<R> a) {
             qs[f] = valid ? qin[f * ng + g] : R(0);
             sQ[f * NT + t] = qs[f];
         }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) sM[k * NT + t] = valid ? a.metric[k * ng + g] : R(0);
         if (valid) {
             const R *mc = matc + (size_t)e * 8;
             rho = mc[2];
@@ -352,6 +376,7 @@ stage_kernel(StageArgs<R> a) {
     for (int f = 0; f < NFC; ++f) lf[f] = 0;
     if (node_thr) {
         const R *src = sF + eb * NFP * NFCP;
+#pragma unroll 4
         for (int k = 0; k < NFP; ++k) {
             const R l = a.LT[k * NP + j];
             const R *row = src + k * NFCP;
@@ -360,12 +385,12 @@ stage_kernel(StageArgs<R> a) {
         }
     }
     // column thread: one column of one element in registers
-    const int ct = t;
-    const bool col_thr = ct < EB * NCOL;
+    constexpr int CTP = S::CTP, NRG = S::NRG, RPG = S::RPG;
+    // CTP is a whole number of waves, so rg is the same on every lane of a
+    // wave; readfirstlane tells the compiler, which keeps the D rows scalar
+    const int rg = __builtin_amdgcn_readfirstlane(t / CTP), ct = t - (t / CTP) * CTP;
+    const bool col_thr = rg < NRG && ct < EB * NCOL;
     const int ceb = col_thr ? ct / NCOL : 0, cc = col_thr ? ct - (ct / NCOL) * NCOL : 0;
-    const int cidx = eblk * EB + ceb;
-    const bool cvalid = col_thr && cidx < a.nlist;
-    const int ce = cvalid ? a.elist[cidx] : 0;
     R X[NP];
     {
         const R *colp;
@@ -383,24 +408,41 @@ stage_kernel(StageArgs<R> a) {
         // which metric row(s) the column needs: PML flux columns one direction,
         // q / velocity columns all three (src/timeloopMod.f90:417-419)
         const int kind = PML ? (cc < 8 ? 0 : (cc < 16 ? 1 : (cc < 24 ? 2 : 3))) : 3;
-        const R *met = a.metric + (size_t)ce * NP;
-        for (int jr = 0; jr < NP; ++jr) {
+        const R *met = sM + ceb * NP;
+        // one row of Dr, Ds and Dt at a time, in chunks of one 64-byte scalar
+        // load per operator; the scheduling barriers keep the compiler from
+        // hoisting every chunk's loads ahead (more values than SGPRs)
+        constexpr int MC = 64 / (int)sizeof(R);
+        constexpr int NCH = (NP + MC - 1) / MC;
+        const int r0 = rg * RPG, r1 = (r0 + RPG < NP) ? r0 + RPG : NP;
+#pragma unroll 1
+        for (int jr = r0; jr < r1; ++jr) {
+            cptr<R> d0 = Dc + jr * NP;
+            cptr<R> d1 = d0 + NP * NP;
+            cptr<R> d2 = d1 + NP * NP;
             R dr = 0, ds = 0, dt = 0;
 #pragma unroll
-            for (int m = 0; m < NP; ++m) {
-                const R x = X[m];
-                dr += Dc[jr * NP + m] * x;
-                ds += Dc[NP * NP + jr * NP + m] * x;
-                dt += Dc[2 * NP * NP + jr * NP + m] * x;
+            for (int c = 0; c < NCH; ++c) {
+#pragma unroll
+                for (int i = 0; i < MC; ++i) {
+                    const int m = c * MC + i;
+                    if (m < NP) {
+                        const R x = X[m];
+                        dr += d0[m] * x;
+                        ds += d1[m] * x;
+                        dt += d2[m] * x;
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
             }
             R *out = sV + (ceb * NP + jr) * NV;
             if (kind < 3) {
-                const R m0 = met[(kind + 0) * ng + jr], m1 = met[(kind + 3) * ng + jr], m2 = met[(kind + 6) * ng + jr];
+                const R m0 = met[(kind + 0) * NT + jr], m1 = met[(kind + 3) * NT + jr], m2 = met[(kind + 6) * NT + jr];
                 out[cc] = m0 * dr + m1 * ds + m2 * dt;
             } else {
-                const R rx = met[0 * ng + jr], ry = met[1 * ng + jr], rz = met[2 * ng + jr];
-                const R sx = met[3 * ng + jr], sy = met[4 * ng + jr], sz = met[5 * ng + jr];
-                const R tx = met[6 * ng + jr], ty = met[7 * ng + jr], tz = met[8 * ng + jr];
+                const R rx = met[0 * NT + jr], ry = met[1 * NT + jr], rz = met[2 * NT + jr];
+                const R sx = met[3 * NT + jr], sy = met[4 * NT + jr], sz = met[5 * NT + jr];
+                const R tx = met[6 * NT + jr], ty = met[7 * NT + jr], tz = met[8 * NT + jr];
                 const int base = PML ? 24 + (cc - 24) * 3 : cc * 3;
                 out[base + 0] = rx * dr + sx * ds + tx * dt;
                 out[base + 1] = ry * dr + sy * ds + ty * dt;

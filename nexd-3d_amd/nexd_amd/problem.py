@@ -80,14 +80,18 @@ class Problem:
 
 
 def setup(mesh: meshmod.TetMesh, order: int, par: dict, sources, receivers,
-          name: str = "", full_source_arrays: bool = True) -> Problem:
+          name: str = "", full_source_arrays: bool = True, dt: float | None = None) -> Problem:
+    """Everything the time loop needs.  ``dt`` replaces the step the CFL
+    formula gives (src/meshMod.f90 dtfactor times cfl) when a run has to be
+    matched whose mesher computed it in another precision."""
     p = dict(DEFAULT_PAR)
     p.update(par)
     ref = refelem.build(order)
     mat = physics.materials(mesh.vp, mesh.vs, mesh.rho, mesh.qp, mesh.qs, p["attenuation"],
                             p["f0_att"], p["fr_att"])
     geom = meshmod.build_geometry(mesh, order, mat.vpu)
-    dt = geom.dtfactor * p["cfl"]
+    if dt is None:
+        dt = geom.dtfactor * p["cfl"]
     nt = int(p["nt"])
     src_ops = None
     if sources:
@@ -176,7 +180,7 @@ def from_reference_example(cubit_dir: str, order: int, par: dict, sources, recei
 
 
 def from_mesh_arrays(coord, elem, props, mat, free, absorb, matprop, order: int, par: dict, sources, receivers,
-                     name: str = "") -> Problem:
+                     name: str = "", dt: float | None = None) -> Problem:
     """A set-up from the numbers of the reference's cubit/ files: elem,
     free and absorb rows are 1-based as in the files; props = vp vs rho qp qs
     per element; mat = (id, material, pml) rows."""
@@ -186,12 +190,13 @@ def from_mesh_arrays(coord, elem, props, mat, free, absorb, matprop, order: int,
                         mat=np.asarray(mat)[:, 1].astype(np.int64), pml=np.asarray(mat)[:, 2].astype(np.int64),
                         free=np.asarray(free, np.int64) - 1, absorb=np.asarray(absorb, np.int64) - 1,
                         matprop=np.asarray(matprop, float))
-    return setup(m, order, par, sources, receivers, name=name)
+    return setup(m, order, par, sources, receivers, name=name, dt=dt)
 
 
-def brick_example(path: str, nt: int | None = None) -> Problem:
+def brick_example(path: str, nt: int | None = None, dt: float | None = None) -> Problem:
     """The reference's simulations/brick example (10 m cube, 9,859 tets, N=4,
-    z-force Ricker at 500 Hz, 11 receivers) from the packed fixture."""
+    z-force Ricker at 500 Hz, 11 receivers) from the packed fixture.  ``dt``
+    overrides the CFL step (the shipped run's step is its time column)."""
     d = np.load(path)
     timeint, nt0, cfl, _ = d["par"]
     sx, sy, sz, stype, stf, f0, factor, ax, ay, az = d["source"]
@@ -199,4 +204,4 @@ def brick_example(path: str, nt: int | None = None) -> Problem:
                         angle_force=(ax, ay, az))
     return from_mesh_arrays(d["coord"], d["elem"], d["props"], d["mat"], d["free"], d["absorb"], d["matprop"], 4,
                             dict(timeint=int(timeint), nt=int(nt or nt0), cfl=float(cfl)), [src], d["stations"],
-                            name="brick")
+                            name="brick", dt=dt)
