@@ -33,24 +33,66 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes_pml_att_stage1(np_, npf, w=8):
-    """HBM bytes one PML+attenuation element needs in RK stage 1 (no RK-base
-    reads): per node read q 9, theta 18, f'/g'/h' 27, metric 9, PML
-    coefficients 9 and write q 9, theta 18, f'/g'/h' 27; per element
-    fscale 4*NpF, face frames 36, material 5, anelastic 9 reals and 4*NpF+4
-    neighbour/face-type ints.  Neighbour traces are other elements' q, read
-    once already, so they are not counted."""
-    per_node = (9 + 18 + 27 + 9 + 9) + (9 + 18 + 27)
+def algorithmic_bytes_pml_att(np_, npf, w=8, rk_base=False):
+    """HBM bytes one PML+attenuation element needs in one RK stage: per node
+    read q 9, theta 18, f'/g'/h' 27, metric 9, PML coefficients 9 and write
+    q 9, theta 18, f'/g'/h' 27 (stages after the first also read the
+    step-start q, theta, f'/g'/h': 54 more); per element fscale 4*NpF, face
+    frames 36, material 5, anelastic 9 reals and 4*NpF+4 neighbour/face-type
+    ints.  Neighbour traces are other elements' q, read once already, so they
+    are not counted."""
+    per_node = (9 + 18 + 27 + 9 + 9) + (9 + 18 + 27) + (54 if rk_base else 0)
     per_elem = 4 * npf + 36 + 5 + 9
     return w * (np_ * per_node + per_elem) + 4 * (4 * npf + 4)
 
 
-def algorithmic_bytes_att_stage1(np_, npf, w=8):
+def algorithmic_bytes_att(np_, npf, w=8, rk_base=False):
     """Same for an attenuating element outside the PML: q, theta, metric
-    read; q, theta written."""
-    per_node = (9 + 18 + 9) + (9 + 18)
+    read; q, theta written; the step-start q, theta read after stage 1."""
+    per_node = (9 + 18 + 9) + (9 + 18) + (27 if rk_base else 0)
     per_elem = 4 * npf + 36 + 5 + 9
     return w * (np_ * per_node + per_elem) + 4 * (4 * npf + 4)
+
+
+def stage_mean_bytes(fn, np_, npf, w, nrk=2):
+    """Algorithmic bytes per element averaged over the nrk stages of a step."""
+    return (fn(np_, npf, w, False) + (nrk - 1) * fn(np_, npf, w, True)) / nrk
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_layered_box_n4.json")
+
+
+def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv):
+    """Per-dispatch HBM bytes of each stage kernel from two rocprofv3 --pmc
+    runs (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by what the
+    same counter reads for a plain copy of known size (bench.py --copy-probe)."""
+    import csv
+    from collections import defaultdict
+
+    def per_kernel(path, counter):
+        vals = defaultdict(list)
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] == counter:
+                vals[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+        return vals
+
+    def copy_scale(path, counter):
+        v = per_kernel(path, counter)
+        k = [n for n in v if "copyBuffer" in n or "copy" in n.lower()]
+        kib = sum(v[k[0]]) / len(v[k[0]])
+        return (1024 * 1024 * 1024 / 1024) / kib   # 1 GiB copy = 1,048,576 KiB each way
+
+    fs, ws = copy_scale(copy_fetch_csv, "FETCH_SIZE"), copy_scale(copy_write_csv, "WRITE_SIZE")
+    f, wr = per_kernel(fetch_csv, "FETCH_SIZE"), per_kernel(write_csv, "WRITE_SIZE")
+    out = {"fetch_scale": fs, "write_scale": ws, "kernels": {}}
+    for name in f:
+        if "stage_kernel" not in name or name not in wr:
+            continue
+        rd = sum(f[name]) / len(f[name]) * 1024 * fs
+        wt = sum(wr[name]) / len(wr[name]) * 1024 * ws
+        out["kernels"][name] = {"dispatches": len(f[name]), "read_bytes": rd, "write_bytes": wt,
+                                "bytes_per_launch": rd + wt}
+    return out
 
 
 def cpu_baseline(order=4):
@@ -84,6 +126,39 @@ def rel_l2_max(a, b):
     return float(np.max(num / den))
 
 
+def accuracy_run(args, pml_cells):
+    """Shot 0 for accuracy_nt steps: FP64 plain path against the FP32 batched
+    path with and without the compensated RK sum.  Errors are the maximum
+    over receivers and components of the relative L2 per trace, separately
+    for displacement, velocity and acceleration."""
+    import numpy as np
+    from nexd_amd import batch, capi, layout
+    nt = args.accuracy_nt
+    sb = batch.layered_shots(1, nx=args.nx, ny=args.ny, nz=args.nz, order=4, nt=nt, pml_cells=pml_cells,
+                             nrec_side=8, timeint=2, attenuation=True, full_source_arrays=True)
+    t0 = time.time()
+    with capi.Context(layout.reference_arrays(sb.shots[0]), compat=capi.COMPAT_ALL) as ctx:
+        ctx.reset()
+        ctx.run(1, nt)
+        ref = ctx.seismograms()
+    out = {"steps": nt, "fp64_wall_s": time.time() - t0}
+    for comp in (False, True):
+        t0 = time.time()
+        with capi.BatchContext(sb, precision=capi.NEXD_FP32, compat=capi.COMPAT_ALL, compensated=comp) as b:
+            b.reset()
+            b.run(1, nt)
+            got = b.seismograms(0)
+        num = np.linalg.norm(got - ref, axis=-1)
+        den = np.maximum(np.linalg.norm(ref, axis=-1), 1e-300)
+        err = num / den
+        key = "fp32_compensated" if comp else "fp32"
+        out[key] = {"u": float(err[:, 0:3].max()), "v": float(err[:, 3:6].max()), "a": float(err[:, 6:9].max()),
+                    "wall_s": time.time() - t0}
+        log(f"[bench] accuracy over {nt} steps, {key}: u {out[key]['u']:.3e} v {out[key]['v']:.3e} "
+            f"a {out[key]['a']:.3e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,9 +170,19 @@ def main():
     ap.add_argument("--nz", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--accuracy-nt", type=int, default=0, metavar="NT",
+                    help="also run shot 0 for NT steps in FP64 and in FP32 (plain and compensated) and report "
+                         "the seismogram error of each FP32 form against FP64")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (calibrates HBM byte counters)")
+    ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
+                    help="only summarise rocprofv3 counter runs into per-launch HBM bytes (JSON on stdout)")
     args = ap.parse_args()
+    if args.pmc_summary:
+        summ = pmc_summary(*args.pmc_summary)
+        summ["workload"] = f"{args.nx}x{args.ny}x{args.nz}/{args.shots}"
+        print(json.dumps(summ, indent=1))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
@@ -137,10 +222,10 @@ def main():
         ctx.run(args.warmup + 1, nsteps_all)
         wall64 = time.perf_counter() - t1
         seis64 = ctx.seismograms()[:, :, :nsteps_all]
-        ms_pml, n_pml = ctx.time_stage_kernel(1, nrep=20)
-        ms_plain, n_plain = ctx.time_stage_kernel(0, nrep=20)
+        ms_pml, n_pml = ctx.time_stage_kernel(1, nrep=10)
+        ms_plain, n_plain = ctx.time_stage_kernel(0, nrep=10)
         ctx.close()
-        b_pml = algorithmic_bytes_pml_att_stage1(p.ref.np_, p.ref.npf) * n_pml
+        b_pml = stage_mean_bytes(algorithmic_bytes_pml_att, p.ref.np_, p.ref.npf, 8) * n_pml
         fp64 = dict(value=dof_step * args.steps / wall64, ms_per_step=wall64 / args.steps * 1e3,
                     ms_pml=ms_pml, n_pml=n_pml, ms_plain=ms_plain, n_plain=n_plain,
                     achieved=b_pml / (ms_pml * 1e-3) / 1e9)
@@ -156,16 +241,33 @@ def main():
     bctx.run(args.warmup + 1, nsteps_all)
     wall32 = time.perf_counter() - t1
     seis32 = bctx.seismograms(0)[:, :, :nsteps_all]
-    ms32_pml, n32_pml = bctx.time_stage_kernel(1, nrep=10)
+    ms32_pml, n32_pml = bctx.time_stage_kernel(1, nrep=5)
     bctx.close()
     value32 = dof_step * sb.nshot * args.steps / wall32
-    b32 = algorithmic_bytes_pml_att_stage1(p.ref.np_, p.ref.npf, w=4) * n32_pml * sb.nshot
+    b32 = stage_mean_bytes(algorithmic_bytes_pml_att, p.ref.np_, p.ref.npf, 4) * n32_pml * sb.nshot
     achieved32 = b32 / (ms32_pml * 1e-3) / 1e9
     err32 = rel_l2_max(seis32, seis64) if fp64 else None
     log(f"[bench] FP32 batched ({sb.nshot} shots): {wall32 / args.steps * 1e3:.2f} ms/step, PML stage kernel "
         f"{ms32_pml:.3f} ms ({achieved32:.0f} GB/s), seismogram error vs FP64 over {nsteps_all} steps: {err32}")
 
+    accuracy = None
+    if args.accuracy_nt:
+        accuracy = accuracy_run(args, pml_cells)
     cpu = None if args.no_cpu_baseline else cpu_baseline()
+    # HBM bytes per launch measured by rocprofv3 counters on this workload
+    # (committed summary, made with --pmc-summary); null when absent
+    traffic = traffic64 = None
+    workload_key = f"{args.nx}x{args.ny}x{args.nz}/{sb.nshot}"
+    if os.path.exists(TRAFFIC_FILE):
+        tf = json.load(open(TRAFFIC_FILE))
+        if tf.get("workload") == workload_key:
+            for name, v in tf["kernels"].items():
+                if "<4, true, true, float>" in name:
+                    traffic = v["bytes_per_launch"]
+                elif "<4, true, true, double>" in name:
+                    traffic64 = v["bytes_per_launch"]
+    bpe32 = stage_mean_bytes(algorithmic_bytes_pml_att, p.ref.np_, p.ref.npf, 4)
+    bpe64 = stage_mean_bytes(algorithmic_bytes_pml_att, p.ref.np_, p.ref.npf, 8)
     out = {
         "metric": "DOF-updates/s on 1 GPU: FP64 plain vs FP32 batched at ≤1e-4 seismogram L2",
         "value": value32,
@@ -187,15 +289,17 @@ def main():
                    "fp64_plain_ms_per_step": fp64["ms_per_step"] if fp64 else None,
                    "fp32_over_fp64_shot_throughput": (value32 / fp64["value"]) if fp64 else None,
                    "fp32_seismogram_rel_l2_vs_fp64": err32,
-                   "accuracy_steps": nsteps_all},
+                   "accuracy_steps": nsteps_all,
+                   "accuracy": accuracy},
         "roofline": {"bound": "hbm", "achieved": achieved32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved32 / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "stage_kernel<4,ATT,PML,f32> (RK stage 1, PML elements, all shots)",
-                     "bytes_per_element": algorithmic_bytes_pml_att_stage1(p.ref.np_, p.ref.npf, w=4),
+                     "frac": achieved32 / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "stage_kernel<4,ATT,PML,f32> (PML elements, all shots; mean over the 2 RK stages)",
+                     "bytes_per_element": bpe32, "algorithmic_bytes_per_launch": bpe32 * n32_pml * sb.nshot,
                      "avg_launch_ms": ms32_pml, "elements_per_launch": n32_pml * sb.nshot,
+                     "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if traffic else None,
                      "fp64_kernel": ({"achieved": fp64["achieved"], "frac": fp64["achieved"] / HBM_PEAK_GBS,
                                       "avg_launch_ms": fp64["ms_pml"], "elements_per_launch": fp64["n_pml"],
-                                      "bytes_per_element": algorithmic_bytes_pml_att_stage1(p.ref.np_, p.ref.npf)}
+                                      "bytes_per_element": bpe64, "traffic": traffic64}
                                      if fp64 else None)},
         "cpu_baseline": cpu,
     }

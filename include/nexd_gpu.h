@@ -143,13 +143,13 @@ int nexd_get_state(nexd_ctx *ctx, double *q, double *ux, double *uy, double *uz)
 int nexd_last_run_ms(nexd_ctx *ctx, double *ms);
 
 /* Measurement only (no reference counterpart; the reference times whole
- * steps with system_clock, src/timeloopMod.f90:544-566): launches the first
- * RK stage kernel of one element list nrep times on the context stream
- * between HIP events and returns the mean duration of one launch and the
- * number of elements it advances.  list 0: elements outside the PML (all
- * elements without PML), list 1: PML elements.  Stage 1 reads only the
- * current state and writes the stage buffer, so repeating it leaves the
- * state unchanged. */
+ * steps with system_clock, src/timeloopMod.f90:544-566): launches the stage
+ * kernel of one element list for every RK stage in turn, nrep rounds, on the
+ * context stream between HIP events, and returns the mean duration of one
+ * launch (averaged over the stages, as in the time loop) and the number of
+ * elements it advances.  list 0: elements outside the PML (all elements
+ * without PML), list 1: PML elements.  The stages overwrite the state: call
+ * nexd_reset before running again.  Needs a multi-stage scheme. */
 int nexd_time_stage_kernel(nexd_ctx *ctx, int list, int nrep, double *avg_ms, int *nelem_launch);
 
 void nexd_destroy(nexd_ctx *ctx);
@@ -177,6 +177,8 @@ typedef struct nexd_batch_config {
     int nt, timeint, attenuation, pml, compat;
     int precision;        /* NEXD_FP32 or NEXD_FP64 */
     int shared_material;  /* 1: one material/PML set for every shot */
+    int compensated;      /* 1: compensated RK stage sum (state kept as hi+lo
+                             pairs of the working precision) */
     int device;
     double dt;
 } nexd_batch_config;
@@ -214,7 +216,9 @@ int nexd_batch_run(nexd_batch *b, int it_begin, int it_end);
 int nexd_batch_get_seismograms(nexd_batch *b, int shot, double *ux, double *uy, double *uz,
                                double *vx, double *vy, double *vz, double *ax, double *ay, double *az);
 int nexd_batch_last_run_ms(nexd_batch *b, double *ms);
-/* measurement: mean duration of one RK-stage-1 launch of element list 0/1 */
+/* measurement: mean duration of one stage launch of element list 0/1,
+ * averaged over the RK stages (as nexd_time_stage_kernel); overwrites the
+ * state */
 int nexd_batch_time_stage_kernel(nexd_batch *b, int list, int nrep, double *avg_ms, int *nelem_launch);
 void nexd_batch_destroy(nexd_batch *b);
 const char *nexd_batch_last_error(const nexd_batch *b);

@@ -39,6 +39,8 @@ struct Batch : BatchBase {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<void *> allocs;
     R *q[3] = {nullptr, nullptr, nullptr};
+    R *qlo[3] = {nullptr, nullptr, nullptr};   // low parts (compensated state)
+    R *u_slot_lo = nullptr;
     R *th[2] = {nullptr, nullptr};
     R *paux[2] = {nullptr, nullptr};
     R *D = nullptr, *LT = nullptr, *mass = nullptr, *metric = nullptr, *fscale = nullptr, *frame = nullptr;
@@ -125,6 +127,7 @@ struct Batch : BatchBase {
         const size_t S = (size_t)c.nshot;
         int rc = 0;
         for (int i = 0; i < 3 && !rc; ++i) rc = dalloc(&q[i], S * 9 * ng);
+        for (int i = 0; i < 3 && !rc && c.compensated; ++i) rc = dalloc(&qlo[i], S * 9 * ng);
         if (!rc && c.attenuation) (rc = dalloc(&th[0], S * 18 * ng)) || (rc = dalloc(&th[1], S * 18 * ng));
         if (!rc && c.pml)
             (rc = dalloc(&paux[0], S * 27 * ng)) || (rc = dalloc(&paux[1], S * 27 * ng)) ||
@@ -281,6 +284,11 @@ struct Batch : BatchBase {
         a.qin = qin; a.qbase = q[0]; a.qout = qout;
         a.thin = thin; a.thbase = th[0]; a.thout = thout;
         a.pin = pin; a.pbase = paux[0]; a.pout = pout;
+        if (cfg.compensated) {
+            auto lo = [&](const R *p) { return p == q[0] ? qlo[0] : (p == q[1] ? qlo[1] : qlo[2]); };
+            a.qlo_in = lo(qin); a.qlo_base = qlo[0]; a.qlo_out = lo(qout);
+            a.u_slot_lo = u_slot_lo;
+        }
     }
 
     void enqueue_step() {
@@ -297,8 +305,12 @@ struct Batch : BatchBase {
                 a.nlist = n_pml;
                 dispatch(a, true);
             }
-            if (cfg.timeint == NEXD_EULER)
+            if (cfg.timeint == NEXD_EULER) {
                 (void)hipMemcpyAsync(q[0], q[1], sizeof(R) * 9 * ng * cfg.nshot, hipMemcpyDeviceToDevice, stream);
+                if (cfg.compensated)
+                    (void)hipMemcpyAsync(qlo[0], qlo[1], sizeof(R) * 9 * ng * cfg.nshot, hipMemcpyDeviceToDevice,
+                                         stream);
+            }
         }
         step_end(1, 1);
     }
@@ -317,6 +329,7 @@ struct Batch : BatchBase {
         for (int i = 0; i < 3; ++i) {
             int rc = put(q[i], std::vector<R>(S * 9 * ng, tiny));
             if (rc) return rc;
+            if (qlo[i] && (rc = put(qlo[i], std::vector<R>(S * 9 * ng, R(0))))) return rc;
         }
         if (cfg.attenuation)
             for (int i = 0; i < 2; ++i) {
@@ -332,6 +345,7 @@ struct Batch : BatchBase {
             std::vector<R> t(S * nslot * 3 * np, tiny);
             int rc;
             if ((rc = put(u_slot, t)) || (rc = put(a_slot, t))) return rc;
+            if (u_slot_lo && (rc = put(u_slot_lo, std::vector<R>(S * nslot * 3 * np, R(0))))) return rc;
         }
         int rc = put(seis, std::vector<R>(S * 9 * cfg.nt * (cfg.nrec > 0 ? cfg.nrec : 1), R(0)));
         if (rc) return rc;
@@ -383,17 +397,24 @@ struct Batch : BatchBase {
         const bool pml = cfg.pml && n_pml > 0;
         if (list == 1 && !pml) return fail(NEXD_E_STATE, "no PML element list");
         nexd::StageArgs<R> a = base_args();
-        set_stage_buffers(a, 1);
         a.elist = list == 0 ? elist_plain : elist_pml;
         a.nlist = list == 0 ? n_plain : n_pml;
+        // every RK stage in turn, as the time loop runs them, after a stage-1
+        // warm-up; the last element's fluxes are computed once beforehand
+        set_stage_buffers(a, 1);
         if (list == 1 && a.pml_compat) last_flux_dispatch(a.qin, a.pin);
         dispatch(a, list == 1);
         if (hipEventRecord(ev0, stream) != hipSuccess) return fail(NEXD_E_HIP, "hipEventRecord failed");
-        for (int r = 0; r < nrep; ++r) dispatch(a, list == 1);
+        for (int r = 0; r < nrep; ++r)
+            for (int irk = 1; irk <= nrk; ++irk) {
+                set_stage_buffers(a, irk);
+                dispatch(a, list == 1);
+            }
         if (hipEventRecord(ev1, stream) != hipSuccess) return fail(NEXD_E_HIP, "hipEventRecord failed");
         if (hipStreamSynchronize(stream) != hipSuccess) return fail(NEXD_E_HIP, "hipStreamSynchronize failed");
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, ev0, ev1);
+        nrep *= nrk;
         *avg_ms = ms / nrep;
         if (nel) *nel = a.nlist;
         return 0;
@@ -559,6 +580,7 @@ int upload_receivers(Batch<R> *b, const int *recelem, const double *rec_int) {
     int rc;
     const size_t S = b->cfg.nshot;
     if ((rc = b->dalloc(&b->u_slot, S * nslot * 3 * np)) || (rc = b->dalloc(&b->a_slot, S * nslot * 3 * np))) return rc;
+    if (b->cfg.compensated && (rc = b->dalloc(&b->u_slot_lo, S * nslot * 3 * np))) return rc;
     b->nslot = nslot;
     std::vector<R> w(rec_int, rec_int + (size_t)np * nr);
     if ((rc = b->put(b->rec_slot, slot_of_elem)) || (rc = b->upload(&b->rec_slot_of, slot_of_rec)) ||

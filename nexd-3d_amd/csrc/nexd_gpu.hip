@@ -232,6 +232,28 @@ StageArgs<double> base_args(nexd_ctx *c) {
 }
 
 // enqueue one full time step on the context stream
+// buffer roles of stage irk: RK2 ping-pongs Q0 -> Q1 -> Q0, RK3 runs
+// Q0 -> Q1 -> Q2 -> Q0; the step-start state Q0 is the RK base throughout
+void stage_buffers(nexd_ctx *c, int irk, StageArgs<double> &a) {
+    const int scheme = c->cfg.timeint;
+    double *qin, *qout, *thin, *thout, *pin, *pout;
+    if (scheme == NEXD_EULER) {
+        qin = c->q[0]; qout = c->q[1]; thin = c->th[0]; thout = c->th[0]; pin = c->paux[0]; pout = c->paux[0];
+    } else if (scheme == NEXD_RK2) {
+        if (irk == 1) { qin = c->q[0]; qout = c->q[1]; thin = c->th[0]; thout = c->th[1]; pin = c->paux[0]; pout = c->paux[1]; }
+        else { qin = c->q[1]; qout = c->q[0]; thin = c->th[1]; thout = c->th[0]; pin = c->paux[1]; pout = c->paux[0]; }
+    } else {
+        if (irk == 1) { qin = c->q[0]; qout = c->q[1]; thin = c->th[0]; thout = c->th[1]; pin = c->paux[0]; pout = c->paux[1]; }
+        else if (irk == 2) { qin = c->q[1]; qout = c->q[2]; thin = c->th[1]; thout = c->th[1]; pin = c->paux[1]; pout = c->paux[1]; }
+        else { qin = c->q[2]; qout = c->q[0]; thin = c->th[1]; thout = c->th[0]; pin = c->paux[1]; pout = c->paux[0]; }
+    }
+    a.irk = irk;
+    a.last_stage = irk == c->nrk ? 1 : 0;
+    a.qin = qin; a.qbase = c->q[0]; a.qout = qout;
+    a.thin = thin; a.thbase = c->th[0]; a.thout = thout;
+    a.pin = pin; a.pbase = c->paux[0]; a.pout = pout;
+}
+
 void enqueue_step(nexd_ctx *c) {
     const int scheme = c->cfg.timeint;
     const bool att = c->cfg.attenuation != 0;
@@ -239,23 +261,8 @@ void enqueue_step(nexd_ctx *c) {
     StageArgs<double> a = base_args(c);
     const int nrk = c->nrk;
     for (int irk = 1; irk <= nrk; ++irk) {
-        double *qin, *qbase = c->q[0], *qout;
-        double *thin, *thout, *pin, *pout;
-        if (scheme == NEXD_EULER) {
-            qin = c->q[0]; qout = c->q[1]; thin = c->th[0]; thout = c->th[0]; pin = c->paux[0]; pout = c->paux[0];
-        } else if (scheme == NEXD_RK2) {
-            if (irk == 1) { qin = c->q[0]; qout = c->q[1]; thin = c->th[0]; thout = c->th[1]; pin = c->paux[0]; pout = c->paux[1]; }
-            else { qin = c->q[1]; qout = c->q[0]; thin = c->th[1]; thout = c->th[0]; pin = c->paux[1]; pout = c->paux[0]; }
-        } else {
-            if (irk == 1) { qin = c->q[0]; qout = c->q[1]; thin = c->th[0]; thout = c->th[1]; pin = c->paux[0]; pout = c->paux[1]; }
-            else if (irk == 2) { qin = c->q[1]; qout = c->q[2]; thin = c->th[1]; thout = c->th[1]; pin = c->paux[1]; pout = c->paux[1]; }
-            else { qin = c->q[2]; qout = c->q[0]; thin = c->th[1]; thout = c->th[0]; pin = c->paux[1]; pout = c->paux[0]; }
-        }
-        a.irk = irk;
-        a.last_stage = irk == nrk ? 1 : 0;
-        a.qin = qin; a.qbase = qbase; a.qout = qout;
-        a.thin = thin; a.thbase = c->th[0]; a.thout = thout;
-        a.pin = pin; a.pbase = c->paux[0]; a.pout = pout;
+        stage_buffers(c, irk, a);
+        const double *qin = a.qin, *pin = a.pin;
         if (pml && a.pml_compat) dispatch_last_flux(c, qin, pin);
         const bool energy = a.last_stage && c->cfg.energy && pml;
         // elements without PML (every element when PML is off)
@@ -614,27 +621,33 @@ int nexd_time_stage_kernel(nexd_ctx *c, int list, int nrep, double *avg_ms, int 
     if (!c->initialized) return fail(c, NEXD_E_STATE, "run or reset before timing");
     const bool pml = c->have_pml && c->pml_active && c->n_pml > 0;
     if (list == 1 && !pml) return fail(c, NEXD_E_STATE, "no PML element list");
+    if (c->nrk == 1) return fail(c, NEXD_E_UNSUPPORTED, "timing needs a multi-stage scheme");
     StageArgs<double> a = base_args(c);
-    a.irk = 1;
-    a.last_stage = c->nrk == 1 ? 1 : 0;
-    a.qin = c->q[0]; a.qbase = c->q[0]; a.qout = c->q[1];
-    a.thin = c->th[0]; a.thbase = c->th[0]; a.thout = c->th[1];
-    a.pin = c->paux[0]; a.pbase = c->paux[0]; a.pout = c->paux[1];
-    if (c->nrk == 1) { a.thout = c->th[1] ? c->th[1] : c->th[0]; a.pout = c->paux[1] ? c->paux[1] : c->paux[0]; a.u_full = nullptr; a.rec_slot = nullptr; }
     a.blk_energy = nullptr;
     a.elist = list == 0 ? c->elist_plain : c->elist_pml;
     a.nlist = list == 0 ? c->n_plain : c->n_pml;
-    if (a.last_stage) return fail(c, NEXD_E_UNSUPPORTED, "timing needs a multi-stage scheme");
     HIPCHK(c, hipSetDevice(c->cfg.device));
+    // every RK stage in turn, as the time loop runs them (stage 1 first as a
+    // warm-up); the receiver-element updates of the last stage are part of
+    // it.  The last element's fluxes are computed once, before the timed
+    // launches: their values do not change the work.
+    const bool att = c->cfg.attenuation != 0;
+    auto one = [&](int irk) {
+        stage_buffers(c, irk, a);
+        dispatch_stage(c->cfg.order, a, att, list == 1, c->stream);
+    };
+    stage_buffers(c, 1, a);
     if (list == 1 && a.pml_compat) dispatch_last_flux(c, a.qin, a.pin);
-    dispatch_stage(c->cfg.order, a, c->cfg.attenuation != 0, list == 1, c->stream);   // warm-up
+    one(1);
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    for (int r = 0; r < nrep; ++r) dispatch_stage(c->cfg.order, a, c->cfg.attenuation != 0, list == 1, c->stream);
+    for (int r = 0; r < nrep; ++r)
+        for (int irk = 1; irk <= c->nrk; ++irk) one(irk);
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipGetLastError());
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    nrep *= c->nrk;
     *avg_ms = ms / nrep;
     if (nelem_launch) *nelem_launch = a.nlist;
     return NEXD_OK;

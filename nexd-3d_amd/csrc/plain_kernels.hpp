@@ -95,6 +95,12 @@ struct StageArgs {
     const R *mass;              // [NP][NP]
     const R *jac;               // [nglob]
     const R *emat;              // [nelem][2] mu, lambda for the energy
+    // compensated state (float pairs hi + lo for the 9 wavefield components
+    // and the receiver-element displacement); null when not compensated
+    const R *qlo_in;            // [9][nglob] low parts of qin
+    const R *qlo_base;          // [9][nglob] low parts of qbase
+    R *qlo_out;                 // [9][nglob]
+    R *u_slot_lo;               // [shot][nslot][3][NP]
 };
 
 // ---------------------------------------------------------------------------
@@ -163,6 +169,60 @@ __device__ __forceinline__ R rk_combine(int scheme, int irk, R a, R base, R rhs,
     if (scheme == 2) return R(0.5) * (base + a + rhs * dt);
     if (irk == 2) return R(0.75) * base + R(0.25) * (a + rhs * dt);
     return w13 * base + w23 * (a + rhs * dt);
+}
+
+// ---------------------------------------------------------------------------
+// Compensated RK combination: the state is carried as an unevaluated pair
+// hi + lo, and the stage sum of rk_combine is formed with error-free
+// transformations (TwoSum: Knuth, TAOCP vol. 2, 4.2.2; FastTwoSum: Dekker
+// 1971; TwoProduct through one FMA), so each step's rounding of the stored
+// state goes into lo instead of being lost.  The right-hand side is computed
+// from hi only; its own rounding is relative to dt*rhs, not to the state.
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ void two_sum(R a, R b, R &s, R &e) {
+#pragma clang fp contract(off)
+    s = a + b;
+    const R bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+template <typename R>
+__device__ __forceinline__ void fast_two_sum(R a, R b, R &s, R &e) {   // |a| >= |b| or a == 0
+#pragma clang fp contract(off)
+    s = a + b;
+    e = b - (s - a);
+}
+template <typename R>
+__device__ __forceinline__ void two_prod(R a, R b, R &p, R &e) {
+#pragma clang fp contract(off)
+    p = a * b;
+    e = __builtin_fma(a, b, -p);
+}
+template <typename R>
+__device__ __forceinline__ void rk_combine_comp(int scheme, int irk, R a, R alo, R base, R blo, R rhs, R dt,
+                                                R w13, R w23, R &hi, R &lo) {
+#pragma clang fp contract(off)
+    const R k = rhs * dt;
+    R s, e, h, l;
+    if (irk == 1) {                                  // a + k
+        two_sum(a, k, s, e);
+        h = s; l = e + alo;
+    } else if (scheme == 2) {                        // (base + a + k) / 2
+        R s1, e1;
+        two_sum(base, a, s1, e1);
+        two_sum(s1, k, s, e);
+        h = R(0.5) * s; l = R(0.5) * ((e1 + e) + (blo + alo));
+    } else {                                         // c1*base + c2*(a + k)
+        const R c1 = irk == 2 ? R(0.75) : w13, c2 = irk == 2 ? R(0.25) : w23;
+        R p1, ep1, p2, ep2;
+        two_sum(a, k, s, e);
+        two_prod(c1, base, p1, ep1);
+        two_prod(c2, s, p2, ep2);
+        two_sum(p1, p2, h, l);
+        l = l + (ep1 + ep2) + (c1 * blo + c2 * (e + alo));
+    }
+    if (h == R(0)) { hi = l; lo = 0; return; }
+    fast_two_sum(h, l, hi, lo);
 }
 
 // elastic fluxes F,G,H of one node (src/waveMod.f90:29-63), 9 columns each
@@ -279,15 +339,20 @@ stage_kernel(StageArgs<R> a) {
     if (node_thr) {
 #pragma unroll
         for (int f = 0; f < 9; ++f) {
-            qs[f] = valid ? qin[f * ng + g] : R(0);
+            const R v = qin[f * ng + g];
+            qs[f] = valid ? v : R(0);
             sQ[f * NT + t] = qs[f];
         }
 #pragma unroll
-        for (int k = 0; k < 9; ++k) sM[k * NT + t] = valid ? a.metric[k * ng + g] : R(0);
+        for (int k = 0; k < 9; ++k) {
+            const R v = a.metric[k * ng + g];
+            sM[k * NT + t] = valid ? v : R(0);
+        }
+        const R *mc = matc + (size_t)e * 8;
+        const R m2 = mc[2], m3 = mc[3], m4 = mc[4];
         if (valid) {
-            const R *mc = matc + (size_t)e * 8;
-            rho = mc[2];
-            ef0 = -(mc[3] + 2 * mc[4]); ef1 = -mc[3]; ef2 = -mc[4]; ef3 = R(-1.0) / rho;
+            rho = m2;
+            ef0 = -(m3 + 2 * m4); ef1 = -m3; ef2 = -m4; ef3 = R(-1.0) / rho;
         }
     }
     if constexpr (PML) {
@@ -302,29 +367,47 @@ stage_kernel(StageArgs<R> a) {
             const R *pbase = a.pbase + (size_t)shot * 27 * ng;
             R *pout = a.pout + (size_t)shot * 27 * ng;
             const R *lastf = a.last_flux + (size_t)shot * 27 * NP;
-            R kk[3] = {1, 1, 1}, al[3] = {0, 0, 0}, dk[3] = {0, 0, 0};
-            if (valid) {
+            R kk[3], al[3], dk[3];
 #pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    al[d] = pmlc[d * ng + g]; dk[d] = pmlc[(3 + d) * ng + g]; kk[d] = pmlc[(6 + d) * ng + g];
-                }
+            for (int d = 0; d < 3; ++d) {
+                const R v0 = pmlc[d * ng + g], v1 = pmlc[(3 + d) * ng + g], v2 = pmlc[(6 + d) * ng + g];
+                al[d] = valid ? v0 : R(0); dk[d] = valid ? v1 : R(0); kk[d] = valid ? v2 : R(1);
             }
             int col = 0;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 R *FF = d == 0 ? Fm : (d == 1 ? Gm : Hm);
                 const int skip = d == 0 ? 4 : (d == 1 ? 5 : 3);
+                // one direction's auxiliaries, last-element fluxes and RK
+                // base are loaded together before any store
+                R cur[9], use[9], base[9];
 #pragma unroll
                 for (int i = 0; i < 9; ++i) {
-                    const int c = d * 9 + i;
-                    const R cur = valid ? pin[c * ng + g] : R(0);
-                    const R mod = (cur + FF[i]) / kk[d];
+                    const R v = pin[(d * 9 + i) * ng + g];
+                    cur[i] = valid ? v : R(0);
+                }
+                if (a.pml_compat) {
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) use[i] = lastf[(d * 9 + i) * NP + j];
+                }
+                if (a.irk != 1) {
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) {
+                        const R v = pbase[(d * 9 + i) * ng + g];
+                        base[i] = valid ? v : R(0);
+                    }
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) base[i] = cur[i];
+                }
+#pragma unroll
+                for (int i = 0; i < 9; ++i) {
+                    const R mod = (cur[i] + FF[i]) / kk[d];
                     if (i != skip) sX[(col++) * NT + t] = mod;
-                    const R use = a.pml_compat ? lastf[c * NP + j] : mod;
-                    const R dadt = -al[d] * cur - dk[d] * (cur + use);
-                    const R base = (a.irk == 1 || !valid) ? cur : pbase[c * ng + g];
-                    const R v = rk_combine<R>(a.scheme, a.irk, cur, base, dadt, a.dt, a.w13, a.w23);
-                    if (valid) pout[c * ng + g] = v;
+                    const R u = a.pml_compat ? use[i] : mod;
+                    const R dadt = -al[d] * cur[i] - dk[d] * (cur[i] + u);
+                    const R v = rk_combine<R>(a.scheme, a.irk, cur[i], base[i], dadt, a.dt, a.w13, a.w23);
+                    if (valid) pout[(d * 9 + i) * ng + g] = v;
                 }
             }
         }
@@ -345,14 +428,15 @@ stage_kernel(StageArgs<R> a) {
         const int fi = k / NPF, i = k - fi * NPF;
         const int own = fb * NP + a.fmask[fi * NPF + i];
         const int ft = a.ftype[fe * 4 + fi];
+        // the neighbour trace is loaded unconditionally (node 0 stands in on
+        // boundary faces) so the nine loads issue together
+        const int nb = a.nbr[((size_t)fe * 4 + fi) * NPF + i];
+        const size_t gn = nb >= 0 ? (size_t)nb : 0;
         R dq[9];
-        if (ft >= 0) {
-            const size_t gn = (size_t)a.nbr[((size_t)fe * 4 + fi) * NPF + i];
 #pragma unroll
-            for (int f = 0; f < 9; ++f) dq[f] = qin[f * ng + gn] - sQ[f * NT + own];
-        } else {
-#pragma unroll
-            for (int f = 0; f < 9; ++f) dq[f] = sQ[f * NT + own];
+        for (int f = 0; f < 9; ++f) {
+            const R qn = qin[f * ng + gn], qo = sQ[f * NT + own];
+            dq[f] = ft >= 0 ? qn - qo : qo;
         }
         const R *mc = matc + (size_t)fe * 8;
         R fl[9], af[6];
@@ -398,7 +482,10 @@ stage_kernel(StageArgs<R> a) {
         else colp = sQ + cc * NT;
         colp += ceb * NP;
 #pragma unroll
-        for (int m = 0; m < NP; ++m) X[m] = col_thr ? colp[m] : R(0);
+        for (int m = 0; m < NP; ++m) {
+            const R v = colp[m];
+            X[m] = col_thr ? v : R(0);
+        }
     }
     __syncthreads();   // sQ/sX/sF are dead from here; sV aliases them
 
@@ -453,6 +540,46 @@ stage_kernel(StageArgs<R> a) {
     __syncthreads();
 
     // ---- assemble the right-hand side (node threads) -----------------------
+    // the memory variables, anelastic coefficients and RK bases this phase
+    // needs are loaded first, in one batch, before any of its stores
+    R th[ATT ? 18 : 1], thb[ATT ? 18 : 1], avc[ATT ? 9 : 1], qb[9];
+    if (node_thr) {
+        if constexpr (ATT) {
+            const R *thin = a.thin + (size_t)shot * 18 * ng;
+            const R *thbase = a.thbase + (size_t)shot * 18 * ng;
+            const R *av = a.anel + (size_t)shot * a.anel_stride + (size_t)e * NMB * 3;
+#pragma unroll
+            for (int c = 0; c < 18; ++c) {
+                const R v = thin[c * ng + g];
+                th[c] = valid ? v : R(0);
+            }
+            if (a.irk == 1) {
+#pragma unroll
+                for (int c = 0; c < 18; ++c) thb[c] = th[c];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 18; ++c) {
+                    const R v = thbase[c * ng + g];
+                    thb[c] = valid ? v : R(0);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 9; ++c) {
+                const R v = av[c];
+                avc[c] = valid ? v : R(0);
+            }
+        }
+        if (a.irk == 1) {
+#pragma unroll
+            for (int f = 0; f < 9; ++f) qb[f] = qs[f];
+        } else {
+#pragma unroll
+            for (int f = 0; f < 9; ++f) {
+                const R v = qbase[f * ng + g];
+                qb[f] = valid ? v : R(0);
+            }
+        }
+    }
     R rhs[9];
     R gv[9];   // velocity gradients: gv[c*3+k] = d v_c / d x_k
     if (node_thr) {
@@ -492,31 +619,26 @@ stage_kernel(StageArgs<R> a) {
         if (node_thr) {
             const R ex = gv[0], ey = gv[4], ez = gv[8];
             const R exy = gv[3] + gv[1], eyz = gv[7] + gv[5], exz = gv[6] + gv[2];
-            const R *av = a.anel + (size_t)shot * a.anel_stride + (size_t)e * NMB * 3;
-            const R *thin = a.thin + (size_t)shot * 18 * ng;
-            const R *thbase = a.thbase + (size_t)shot * 18 * ng;
             R *thout = a.thout + (size_t)shot * 18 * ng;
 #pragma unroll
             for (int i = 0; i < NMB; ++i) {
                 const R w = a.wl[i];
                 const R vol[6] = {-w * ex, -w * ey, -w * ez, R(-0.5) * w * exy, R(-0.5) * w * eyz, R(-0.5) * w * exz};
-                R th[6];
-#pragma unroll
-                for (int c = 0; c < 6; ++c) th[c] = valid ? thin[(i * 6 + c) * ng + g] : R(0);
 #pragma unroll
                 for (int c = 0; c < 6; ++c) {
-                    const R arq = vol[c] + w * lf[9 + c] - w * th[c];
-                    const R base = (a.irk == 1 || !valid) ? th[c] : thbase[(i * 6 + c) * ng + g];
-                    const R nv = rk_combine<R>(a.scheme, a.irk, th[c], base, arq, a.dt, a.w13, a.w23);
-                    if (valid) thout[(i * 6 + c) * ng + g] = nv;
+                    const int k = i * 6 + c;
+                    const R arq = vol[c] + w * lf[9 + c] - w * th[k];
+                    const R nv = rk_combine<R>(a.scheme, a.irk, th[k], thb[k], arq, a.dt, a.w13, a.w23);
+                    if (valid) thout[k * ng + g] = nv;
                 }
-                const R a1 = valid ? av[i * 3 + 0] : R(0), a2 = valid ? av[i * 3 + 1] : R(0), a3 = valid ? av[i * 3 + 2] : R(0);
-                rhs[0] = rhs[0] + a3 * th[0] + a1 * (th[1] + th[2]);
-                rhs[1] = rhs[1] + a3 * th[1] + a1 * (th[2] + th[0]);
-                rhs[2] = rhs[2] + a3 * th[2] + a1 * (th[0] + th[1]);
-                rhs[3] = rhs[3] + a2 * th[3];
-                rhs[4] = rhs[4] + a2 * th[4];
-                rhs[5] = rhs[5] + a2 * th[5];
+                const R a1 = avc[i * 3 + 0], a2 = avc[i * 3 + 1], a3 = avc[i * 3 + 2];
+                const R *tm = th + i * 6;
+                rhs[0] = rhs[0] + a3 * tm[0] + a1 * (tm[1] + tm[2]);
+                rhs[1] = rhs[1] + a3 * tm[1] + a1 * (tm[2] + tm[0]);
+                rhs[2] = rhs[2] + a3 * tm[2] + a1 * (tm[0] + tm[1]);
+                rhs[3] = rhs[3] + a2 * tm[3];
+                rhs[4] = rhs[4] + a2 * tm[4];
+                rhs[5] = rhs[5] + a2 * tm[5];
             }
         }
     }
@@ -552,11 +674,33 @@ stage_kernel(StageArgs<R> a) {
             }
         }
         // Runge-Kutta update
+        if (a.qlo_out) {
+            const size_t so = (size_t)shot * 9 * ng;
+            R alo[9], blo[9];
 #pragma unroll
-        for (int f = 0; f < 9; ++f) {
-            const R base = (a.irk == 1 || !valid) ? qs[f] : qbase[f * ng + g];
-            qnew[f] = rk_combine<R>(a.scheme, a.irk, qs[f], base, rhs[f], a.dt, a.w13, a.w23);
-            if (valid) qout[f * ng + g] = qnew[f];
+            for (int f = 0; f < 9; ++f) {
+                const R v = a.qlo_in[so + f * ng + g];
+                alo[f] = valid ? v : R(0);
+            }
+#pragma unroll
+            for (int f = 0; f < 9; ++f) {
+                const R v = a.qlo_base[so + f * ng + g];
+                blo[f] = (a.irk == 1 || !valid) ? alo[f] : v;
+            }
+#pragma unroll
+            for (int f = 0; f < 9; ++f) {
+                R lo;
+                rk_combine_comp<R>(a.scheme, a.irk, qs[f], alo[f], qb[f], blo[f], rhs[f], a.dt, a.w13, a.w23, qnew[f],
+                                   lo);
+                if (valid) { qout[f * ng + g] = qnew[f]; a.qlo_out[so + f * ng + g] = lo; }
+            }
+        } else {
+#pragma unroll
+            for (int f = 0; f < 9; ++f) {
+                const R base = qb[f];
+                qnew[f] = rk_combine<R>(a.scheme, a.irk, qs[f], base, rhs[f], a.dt, a.w13, a.w23);
+                if (valid) qout[f * ng + g] = qnew[f];
+            }
         }
     }
 
@@ -573,7 +717,15 @@ stage_kernel(StageArgs<R> a) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     R *u = a.u_slot + (so * 3 + c) * NP + j;
-                    *u = *u + a.dt * qnew[6 + c];
+                    if (a.u_slot_lo) {
+                        // displacement by forward Euler, compensated (Kahan)
+                        R *ul = a.u_slot_lo + (so * 3 + c) * NP + j;
+                        R s, e;
+                        two_sum(*u, a.dt * qnew[6 + c], s, e);
+                        fast_two_sum(s, e + *ul, *u, *ul);
+                    } else {
+                        *u = *u + a.dt * qnew[6 + c];
+                    }
                     a.a_slot[(so * 3 + c) * NP + j] = rhs[6 + c];
                 }
             }

@@ -190,7 +190,7 @@ BATCH_EXPORTS = ["nexd_batch_create", "nexd_batch_upload_operators", "nexd_batch
 class NexdBatchConfig(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in ("nelem", "order", "nshot", "nsrc", "nrec", "nt", "timeint",
                                             "attenuation", "pml", "compat", "precision", "shared_material",
-                                            "device")] + [("dt", ctypes.c_double)]
+                                            "compensated", "device")] + [("dt", ctypes.c_double)]
 
 
 def _load_batch():
@@ -211,7 +211,8 @@ def _load_batch():
 class BatchContext:
     """Many shots on one mesh (nexd_amd.batch.ShotBatch), FP32 or FP64."""
 
-    def __init__(self, sb, precision: int = NEXD_FP32, compat: int = COMPAT_ALL, device: int = 0):
+    def __init__(self, sb, precision: int = NEXD_FP32, compat: int = COMPAT_ALL, device: int = 0,
+                 compensated: bool = False):
         from . import layout
         self.lib = _load_batch()
         base = sb.base
@@ -220,7 +221,8 @@ class BatchContext:
         cfg = NexdBatchConfig(nelem=a["nelem"], order=a["order"], nshot=sb.nshot, nsrc=len(sb.shots[0].sources),
                               nrec=a["nrec"], nt=a["nt"], timeint=a["timeint"], attenuation=a["attenuation"],
                               pml=a["pml"], compat=compat, precision=precision,
-                              shared_material=int(sb.shared_material), device=device, dt=base.dt)
+                              shared_material=int(sb.shared_material), compensated=int(compensated),
+                              device=device, dt=base.dt)
         h = ctypes.c_void_p()
         rc = self.lib.nexd_batch_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != 0:

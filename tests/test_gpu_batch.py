@@ -81,3 +81,18 @@ def test_fp32_batch_error_vs_fp64(which, cube_shots, layered_shots):
         print(f"  {which} shot {k}: max rel L2 FP32 vs FP64 = {err.max():.3e} (v: {err[:, 3:6].max():.3e})",
               flush=True)
         assert err.max() <= 1e-4
+
+
+@pytest.mark.parametrize("which", ["cube", "layered"])
+def test_fp32_compensated_error_vs_fp64(which, cube_shots, layered_shots):
+    sb = cube_shots if which == "cube" else layered_shots
+    with capi.BatchContext(sb, precision=capi.NEXD_FP32, compensated=True) as b:
+        b.reset()
+        b.run(1, sb.base.nt)
+        got = [b.seismograms(k) for k in range(sb.nshot)]
+    plain = batch_run(sb, capi.NEXD_FP32)
+    for k, s in enumerate(sb.shots):
+        ref = fp64_single(s)
+        err, err0 = rel_l2(got[k], ref), rel_l2(plain[k], ref)
+        print(f"  {which} shot {k}: FP32 compensated {err.max():.3e}, plain FP32 {err0.max():.3e}", flush=True)
+        assert err.max() <= 1e-4
