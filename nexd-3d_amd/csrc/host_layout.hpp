@@ -40,6 +40,29 @@ inline std::string convert_operators(int np, int npf, const double *dr, const do
     return "";
 }
 
+// Dr, Ds, Dt in the A-operand order of the 16x16x4 MFMA (row tile rt,
+// k-step ks, lane l -> [rt][ks][l]).  Row tile rt holds nodes 4rt..4rt+3
+// and the three operators plus a zero row; lane l of k-step ks holds tile
+// row l%16 at column 4ks + l/16 (zero beyond Np).  The row order inside a
+// tile makes accumulator element v of every lane the operator-v derivative
+// of one node: node-major (4*node + op) for the FP32 instruction, whose
+// lane holds tile rows 4*(l/16)+v, and operator-major (4*op + node) for the
+// FP64 one, whose lane holds rows 4v + l/16.
+inline std::vector<double> mfma_operator_fragments(const std::vector<double> &D, int np, bool f64) {
+    const int nt = (np + 3) / 4, nk = (np + 3) / 4;
+    std::vector<double> out((size_t)nt * nk * 64, 0.0);
+    for (int rt = 0; rt < nt; ++rt)
+        for (int ks = 0; ks < nk; ++ks)
+            for (int l = 0; l < 64; ++l) {
+                const int i = l % 16, m = 4 * ks + l / 16;
+                const int node = f64 ? i % 4 : i / 4, op = f64 ? i / 4 : i % 4;
+                const int jr = 4 * rt + node;
+                if (op < 3 && jr < np && m < np)
+                    out[((size_t)rt * nk + ks) * 64 + l] = D[((size_t)op * np + jr) * np + m];
+            }
+    return out;
+}
+
 struct MeshHost {
     std::vector<double> metric;  // [9][nglob]
     std::vector<double> jac;     // [nglob]

@@ -65,7 +65,7 @@ struct StageArgs {
     R *pout;
     const R *last_flux;         // [27][NP] fluxes of the last element (compat)
     int pml_compat;
-    const R *D;                 // [3][NP][NP]  D[d][i][m]
+    const R *D;                 // Dr/Ds/Dt as MFMA A-operand fragments [RT][KS][64]
     const R *LT;                // [NFP][NP]    LIFT(j,k) at LT[k][j]
     const int *fmask;           // [4][NPF] 0-based
     const R *metric;            // [9][nglob] rx ry rz sx sy sz tx ty tz
@@ -240,11 +240,11 @@ __device__ __forceinline__ void elastic_flux_node(const R q[9], R ef0, R ef1, R 
 // H without 4 (those are identically zero, src/waveMod.f90:37-62)
 __device__ constexpr int kPmlCols = 24;
 
-// Block shape of one stage kernel instantiation.  Threads play two roles:
-// node threads (element-in-block, node) for loads, the face term, LIFT and
-// the update; column threads (element-in-block, column) for the volume
-// term, where one column of q (or of the PML-modified fluxes) stays in
-// registers while the rows of Dr/Ds/Dt stream through scalar loads.
+// Block shape of one stage kernel instantiation.  Node threads (element-in-
+// block, node) do the loads, the face term, LIFT and the update; for the
+// volume term every wave runs 16x16x4 MFMAs over tiles of the block's
+// columns (q, or the PML-modified fluxes), with the columns' fragments in
+// registers and the operator fragments streamed from L2.
 template <int ORDER, bool ATT, bool PML, typename R>
 struct Shape {
     static constexpr int NP = Dims<ORDER>::NP, NPF = Dims<ORDER>::NPF, NFP = Dims<ORDER>::NFP;
@@ -252,8 +252,8 @@ struct Shape {
     static constexpr int NV = PML ? (ATT ? 33 : 24) : 27;      // volume values per node
     static constexpr int NFC = ATT ? 15 : 9;                    // face values per face node
     static constexpr int NFCP = ATT ? 16 : 10;                  // padded for 16-byte LDS reads
-    // four waves per block: the node phases use EB*NP of them, the column
-    // phase splits its rows over as many wave groups as fit (NRG below)
+    // four waves per block: the node phases use EB*NP threads of them, the
+    // volume term all four
     static constexpr int NTHR = 256;
     // blocks per compute unit the shape is sized for (160 KiB of LDS per CU):
     // two FP64 blocks or four FP32 blocks, i.e. 2 or 4 waves per SIMD
@@ -286,17 +286,33 @@ struct Shape {
     static constexpr int NCT = EB * NCOL;
     static constexpr int LDS = lds_reals(EB);
     static constexpr int OFF_M = off_metric(EB);
-    // column threads: CT columns padded to whole waves (CTP); the rows of
-    // the volume product are split into NRG groups, one group per CTP
-    // threads, so every wave works on one row range (uniform operator loads)
-    static constexpr int CT = NCT;
-    static constexpr int CTP = (CT + 63) / 64 * 64;
-    static constexpr int NRG = (NTHR / CTP) > 0 ? NTHR / CTP : 1;
-    static constexpr int RPG = (NP + NRG - 1) / NRG;
+    // volume product on the matrix cores: [Dr;Ds;Dt] (row tiles of 4 nodes
+    // x 4 operators) times the block's columns (tiles of 16), k-steps of 4
+    static constexpr int NW = NTHR / 64;             // waves
+    static constexpr int CTL = (NCT + 15) / 16;      // column tiles
+    static constexpr int CPW = (CTL + NW - 1) / NW;  // column tiles per wave
+    static constexpr int KS = (NP + 3) / 4;          // k-steps
+    static constexpr int RT = (NP + 3) / 4;          // row tiles
 };
 
+// one 16x16x4 MFMA, FP32 or FP64 (the accumulator layouts differ; the
+// operator fragments are ordered to match, see mfma_operator_fragments)
 template <typename R>
-using cptr = const __attribute__((address_space(4))) R *;   // uniform, read-only: scalar loads
+struct Mfma;
+template <>
+struct Mfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 fma(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct Mfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 fma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
 
 template <int ORDER, bool ATT, bool PML, typename R>
 __global__ void __launch_bounds__((Shape<ORDER, ATT, PML, R>::NTHR), (Shape<ORDER, ATT, PML, R>::BPC))
@@ -459,81 +475,116 @@ stage_kernel(StageArgs<R> a) {
 #pragma unroll
     for (int f = 0; f < NFC; ++f) lf[f] = 0;
     if (node_thr) {
+        // LIFT column j in chunks of U face nodes; the next chunk's LIFT
+        // entries load while this one accumulates
+        constexpr int U = NFP % 12 == 0 ? 12 : (NFP % 10 == 0 ? 10 : 4);
+        constexpr int NCH = NFP / U;
         const R *src = sF + eb * NFP * NFCP;
-#pragma unroll 4
-        for (int k = 0; k < NFP; ++k) {
-            const R l = a.LT[k * NP + j];
-            const R *row = src + k * NFCP;
+        const R *lt = a.LT + j;
+        R la[U], lb[U];
 #pragma unroll
-            for (int f = 0; f < NFC; ++f) lf[f] += l * row[f];
+        for (int u = 0; u < U; ++u) la[u] = lt[u * NP];
+#pragma unroll 1
+        for (int c0 = 0; c0 < NCH; c0 += 2) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int c = c0 + h;
+                if (c >= NCH) break;
+                R *lc = h == 0 ? la : lb;
+                R *ln = h == 0 ? lb : la;
+                const int cn = c + 1 < NCH ? c + 1 : c;
+#pragma unroll
+                for (int u = 0; u < U; ++u) ln[u] = lt[(cn * U + u) * NP];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const R *row = src + (c * U + u) * NFCP;
+#pragma unroll
+                    for (int f = 0; f < NFC; ++f) lf[f] += lc[u] * row[f];
+                }
+            }
         }
     }
-    // column thread: one column of one element in registers
-    constexpr int CTP = S::CTP, NRG = S::NRG, RPG = S::RPG;
-    // CTP is a whole number of waves, so rg is the same on every lane of a
-    // wave; readfirstlane tells the compiler, which keeps the D rows scalar
-    const int rg = __builtin_amdgcn_readfirstlane(t / CTP), ct = t - (t / CTP) * CTP;
-    const bool col_thr = rg < NRG && ct < EB * NCOL;
-    const int ceb = col_thr ? ct / NCOL : 0, cc = col_thr ? ct - (ct / NCOL) * NCOL : 0;
-    R X[NP];
-    {
+    // volume operands: each wave takes column tiles w, w+NW, ... of the
+    // block's EB*NCOL columns (a q field, a PML flux column or, with PML and
+    // attenuation, a velocity column) and keeps their B fragments (column
+    // 16*tile + l%16, k = 4*ks + l/16) in registers
+    constexpr int NW = S::NW, CTL = S::CTL, CPW = S::CPW, KS = S::KS, RT = S::RT, NCT = S::NCT;
+    const int lane = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    R xb[CPW][KS];
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+        const int col = (w + c * NW) * 16 + (lane & 15);
+        const bool cok = w + c * NW < CTL && col < NCT;
+        const int ceb = cok ? col / NCOL : 0, cc = cok ? col - (col / NCOL) * NCOL : 0;
         const R *colp;
         if constexpr (PML) colp = cc < 24 ? sX + cc * NT : sQ + (6 + cc - 24) * NT;
         else colp = sQ + cc * NT;
         colp += ceb * NP;
 #pragma unroll
-        for (int m = 0; m < NP; ++m) {
-            const R v = colp[m];
-            X[m] = col_thr ? v : R(0);
+        for (int ks = 0; ks < KS; ++ks) {
+            const int m = ks * 4 + (lane >> 4);
+            const R v = colp[m < NP ? m : 0];
+            xb[c][ks] = (cok && m < NP) ? v : R(0);
         }
     }
     __syncthreads();   // sQ/sX/sF are dead from here; sV aliases them
 
-    // ---- volume term: Dr/Ds/Dt applied to the column, metric per node ------
-    if (col_thr) {
-        cptr<R> Dc = (cptr<R>)a.D;
-        // which metric row(s) the column needs: PML flux columns one direction,
-        // q / velocity columns all three (src/timeloopMod.f90:417-419)
-        const int kind = PML ? (cc < 8 ? 0 : (cc < 16 ? 1 : (cc < 24 ? 2 : 3))) : 3;
-        const R *met = sM + ceb * NP;
-        // one row of Dr, Ds and Dt at a time, in chunks of one 64-byte scalar
-        // load per operator; the scheduling barriers keep the compiler from
-        // hoisting every chunk's loads ahead (more values than SGPRs)
-        constexpr int MC = 64 / (int)sizeof(R);
-        constexpr int NCH = (NP + MC - 1) / MC;
-        const int r0 = rg * RPG, r1 = (r0 + RPG < NP) ? r0 + RPG : NP;
+    // ---- volume term: [Dr;Ds;Dt] x columns on the matrix cores, then the
+    // metric of each node (src/timeloopMod.f90:407-420) -----------------------
+    {
+        using V4 = typename Mfma<R>::v4;
+        const R *dfr = a.D + lane;
+        // operator fragments double-buffered: row tile rt+1 loads while rt
+        // computes (vector loads return in order); two tiles per iteration so
+        // the buffers swap roles without copies
+        R fa[KS], fb[KS];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) fa[ks] = dfr[ks * 64];
 #pragma unroll 1
-        for (int jr = r0; jr < r1; ++jr) {
-            cptr<R> d0 = Dc + jr * NP;
-            cptr<R> d1 = d0 + NP * NP;
-            cptr<R> d2 = d1 + NP * NP;
-            R dr = 0, ds = 0, dt = 0;
+        for (int rt0 = 0; rt0 < RT; rt0 += 2) {
 #pragma unroll
-            for (int c = 0; c < NCH; ++c) {
+            for (int h = 0; h < 2; ++h) {
+                const int rt = rt0 + h;
+                if (rt >= RT) break;
+                R *af = h == 0 ? fa : fb;
+                R *an = h == 0 ? fb : fa;
+                const int rn = rt + 1 < RT ? rt + 1 : rt;   // unconditional: no branch
 #pragma unroll
-                for (int i = 0; i < MC; ++i) {
-                    const int m = c * MC + i;
-                    if (m < NP) {
-                        const R x = X[m];
-                        dr += d0[m] * x;
-                        ds += d1[m] * x;
-                        dt += d2[m] * x;
+                for (int ks = 0; ks < KS; ++ks) an[ks] = dfr[(rn * KS + ks) * 64];
+                const int jr = rt * 4 + (lane >> 4);
+#pragma unroll
+                for (int c = 0; c < CPW; ++c) {
+                    if (w + c * NW >= CTL) break;
+                    V4 acc = {0, 0, 0, 0};
+#pragma unroll
+                    for (int ks = 0; ks < KS; ++ks) acc = Mfma<R>::fma(af[ks], xb[c][ks], acc);
+                    // element v of the accumulator: operator v (Dr, Ds, Dt) of
+                    // node jr, column 16*tile + l%16
+                    const int col = (w + c * NW) * 16 + (lane & 15);
+                    if (col < NCT && jr < NP) {
+                        const int ceb = col / NCOL, cc = col - ceb * NCOL;
+                        const R dr = acc[0], ds = acc[1], dt = acc[2];
+                        // which metric row(s) the column needs: PML flux columns
+                        // one direction, q / velocity columns all three
+                        const int kind = PML ? (cc < 8 ? 0 : (cc < 16 ? 1 : (cc < 24 ? 2 : 3))) : 3;
+                        const R *met = sM + ceb * NP;
+                        R *out = sV + (ceb * NP + jr) * NV;
+                        if (kind < 3) {
+                            const R m0 = met[(kind + 0) * NT + jr], m1 = met[(kind + 3) * NT + jr],
+                                    m2 = met[(kind + 6) * NT + jr];
+                            out[cc] = m0 * dr + m1 * ds + m2 * dt;
+                        } else {
+                            const R rx = met[0 * NT + jr], ry = met[1 * NT + jr], rz = met[2 * NT + jr];
+                            const R sx = met[3 * NT + jr], sy = met[4 * NT + jr], sz = met[5 * NT + jr];
+                            const R tx = met[6 * NT + jr], ty = met[7 * NT + jr], tz = met[8 * NT + jr];
+                            const int base = PML ? 24 + (cc - 24) * 3 : cc * 3;
+                            out[base + 0] = rx * dr + sx * ds + tx * dt;
+                            out[base + 1] = ry * dr + sy * ds + ty * dt;
+                            out[base + 2] = rz * dr + sz * ds + tz * dt;
+                        }
                     }
                 }
-                __builtin_amdgcn_sched_barrier(0);
-            }
-            R *out = sV + (ceb * NP + jr) * NV;
-            if (kind < 3) {
-                const R m0 = met[(kind + 0) * NT + jr], m1 = met[(kind + 3) * NT + jr], m2 = met[(kind + 6) * NT + jr];
-                out[cc] = m0 * dr + m1 * ds + m2 * dt;
-            } else {
-                const R rx = met[0 * NT + jr], ry = met[1 * NT + jr], rz = met[2 * NT + jr];
-                const R sx = met[3 * NT + jr], sy = met[4 * NT + jr], sz = met[5 * NT + jr];
-                const R tx = met[6 * NT + jr], ty = met[7 * NT + jr], tz = met[8 * NT + jr];
-                const int base = PML ? 24 + (cc - 24) * 3 : cc * 3;
-                out[base + 0] = rx * dr + sx * ds + tx * dt;
-                out[base + 1] = ry * dr + sy * ds + ty * dt;
-                out[base + 2] = rz * dr + sz * ds + tz * dt;
             }
         }
     }
