@@ -63,6 +63,22 @@ inline std::vector<double> mfma_operator_fragments(const std::vector<double> &D,
     return out;
 }
 
+// LIFT in the A-operand order of the 16x16x4 MFMA: row tile mt holds nodes
+// 16mt..16mt+15, k-step ks face nodes 4ks..4ks+3; lane l holds node
+// 16mt + l%16 at face node 4ks + l/16 (zero beyond Np).  LT is [NFP][NP]
+// with LT[k][j] = LIFT(j,k).
+inline std::vector<double> mfma_lift_fragments(const std::vector<double> &LT, int np, int nfp) {
+    const int nt = (np + 15) / 16, nk = nfp / 4;
+    std::vector<double> out((size_t)nt * nk * 64, 0.0);
+    for (int mt = 0; mt < nt; ++mt)
+        for (int ks = 0; ks < nk; ++ks)
+            for (int l = 0; l < 64; ++l) {
+                const int i = 16 * mt + l % 16, k = 4 * ks + l / 16;
+                if (i < np) out[((size_t)mt * nk + ks) * 64 + l] = LT[(size_t)k * np + i];
+            }
+    return out;
+}
+
 struct MeshHost {
     std::vector<double> metric;  // [9][nglob]
     std::vector<double> jac;     // [nglob]

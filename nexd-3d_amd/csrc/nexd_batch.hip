@@ -438,7 +438,7 @@ int upload_ops(Batch<R> *b, const double *dr, const double *ds, const double *dt
     const std::string err = nexd::convert_operators(b->np, b->npf, dr, ds, dt, lift, mass, fmask, o);
     if (!err.empty()) return b->fail(NEXD_E_ARG, err);
     int rc;
-    if ((rc = b->upload(&b->D, cvt<R>(nexd::mfma_operator_fragments(o.D, b->np, sizeof(R) == 8)))) || (rc = b->upload(&b->LT, cvt<R>(o.LT))) ||
+    if ((rc = b->upload(&b->D, cvt<R>(nexd::mfma_operator_fragments(o.D, b->np, sizeof(R) == 8)))) || (rc = b->upload(&b->LT, cvt<R>(nexd::mfma_lift_fragments(o.LT, b->np, 4 * b->npf)))) ||
         (rc = b->upload(&b->mass, cvt<R>(o.M))) || (rc = b->upload(&b->fmask, o.fm)))
         return rc;
     b->fmask_host = o.fm;

@@ -66,7 +66,7 @@ struct StageArgs {
     const R *last_flux;         // [27][NP] fluxes of the last element (compat)
     int pml_compat;
     const R *D;                 // Dr/Ds/Dt as MFMA A-operand fragments [RT][KS][64]
-    const R *LT;                // [NFP][NP]    LIFT(j,k) at LT[k][j]
+    const R *LT;                // LIFT as MFMA A-operand fragments [MTL][NFP/4][64]
     const int *fmask;           // [4][NPF] 0-based
     const R *metric;            // [9][nglob] rx ry rz sx sy sz tx ty tz
     const R *fscale;            // [nelem][NFP]
@@ -259,16 +259,23 @@ struct Shape {
     // two FP64 blocks or four FP32 blocks, i.e. 2 or 4 waves per SIMD
     static constexpr int BPC = sizeof(R) == 8 ? 2 : 4;
     static constexpr int LDS_BYTES = 160 * 1024 / BPC;
-    // LDS regions, in reals: q and PML flux columns (a) and face values (b)
-    // until the columns are in registers, then volume values (c) over both;
-    // the metric (m) lives through the volume term; the energy values and the
-    // block reduction alias everything at the end
-    static constexpr int off_metric(int eb) {
+    // LDS regions, in reals: q and PML flux columns (a, from 0) and face
+    // values (b, after a) until the operand fragments are in registers; then
+    // volume values (c, from 0) and the lifted face values (16 per node, after
+    // max(a, c)); the metric (m) lives from the first phase to the assembly,
+    // after all of them; the energy values and the block reduction alias
+    // everything at the end
+    static constexpr int off_lift(int eb) {
         const int nt = eb * NP;
         const int a = (9 + (PML ? 24 : 0)) * nt;
-        const int b = eb * NFP * NFCP;
         const int c = nt * NV;
-        return a + b > c ? a + b : c;
+        return a > c ? a : c;
+    }
+    static constexpr int off_metric(int eb) {
+        const int nt = eb * NP;
+        const int ab = (9 + (PML ? 24 : 0)) * nt + eb * NFP * NFCP;
+        const int l = off_lift(eb) + 16 * nt;
+        return ab > l ? ab : l;
     }
     static constexpr int lds_reals(int eb) {
         const int m = off_metric(eb) + 9 * eb * NP;
@@ -286,6 +293,7 @@ struct Shape {
     static constexpr int NCT = EB * NCOL;
     static constexpr int LDS = lds_reals(EB);
     static constexpr int OFF_M = off_metric(EB);
+    static constexpr int OFF_L = off_lift(EB);
     // volume product on the matrix cores: [Dr;Ds;Dt] (row tiles of 4 nodes
     // x 4 operators) times the block's columns (tiles of 16), k-steps of 4
     static constexpr int NW = NTHR / 64;             // waves
@@ -293,6 +301,11 @@ struct Shape {
     static constexpr int CPW = (CTL + NW - 1) / NW;  // column tiles per wave
     static constexpr int KS = (NP + 3) / 4;          // k-steps
     static constexpr int RT = (NP + 3) / 4;          // row tiles
+    // LIFT on the matrix cores: LIFT (node tiles of 16) times each element's
+    // face values (16 columns: NFC values and padding), k-steps of 4 face nodes
+    static constexpr int MTL = (NP + 15) / 16;       // node tiles
+    static constexpr int KSL = NFP / 4;              // k-steps
+    static constexpr int EPW = (EB + NW - 1) / NW;   // elements per wave
 };
 
 // one 16x16x4 MFMA, FP32 or FP64 (the accumulator layouts differ; the
@@ -326,6 +339,7 @@ stage_kernel(StageArgs<R> a) {
     R *sF = lds + (9 + (PML ? 24 : 0)) * NT;   // [EB][NFP][NFCP]
     R *sV = lds;                          // [NT][NV]    after the columns are in registers
     R *sM = lds + S::OFF_M;               // [9][NT]     metric of the block's nodes
+    R *sL = lds + S::OFF_L;               // [NT][16]    lifted face values
     constexpr int NRED = NTHR <= 64 ? 64 : (NTHR <= 128 ? 128 : 256);   // power of two >= NTHR
     static_assert(NRED == NTHR, "the block reduction assumes a power-of-two block");
     R *sRed = lds + 15 * NT;              // [NTHR]      energy reduction
@@ -470,45 +484,13 @@ stage_kernel(StageArgs<R> a) {
     }
     __syncthreads();
 
-    // ---- LIFT (node threads) and column loads (column threads) -------------
-    R lf[NFC];
-#pragma unroll
-    for (int f = 0; f < NFC; ++f) lf[f] = 0;
-    if (node_thr) {
-        // LIFT column j in chunks of U face nodes; the next chunk's LIFT
-        // entries load while this one accumulates
-        constexpr int U = NFP % 12 == 0 ? 12 : (NFP % 10 == 0 ? 10 : 4);
-        constexpr int NCH = NFP / U;
-        const R *src = sF + eb * NFP * NFCP;
-        const R *lt = a.LT + j;
-        R la[U], lb[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) la[u] = lt[u * NP];
-#pragma unroll 1
-        for (int c0 = 0; c0 < NCH; c0 += 2) {
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int c = c0 + h;
-                if (c >= NCH) break;
-                R *lc = h == 0 ? la : lb;
-                R *ln = h == 0 ? lb : la;
-                const int cn = c + 1 < NCH ? c + 1 : c;
-#pragma unroll
-                for (int u = 0; u < U; ++u) ln[u] = lt[(cn * U + u) * NP];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const R *row = src + (c * U + u) * NFCP;
-#pragma unroll
-                    for (int f = 0; f < NFC; ++f) lf[f] += lc[u] * row[f];
-                }
-            }
-        }
-    }
-    // volume operands: each wave takes column tiles w, w+NW, ... of the
-    // block's EB*NCOL columns (a q field, a PML flux column or, with PML and
-    // attenuation, a velocity column) and keeps their B fragments (column
-    // 16*tile + l%16, k = 4*ks + l/16) in registers
+    // ---- operand fragments into registers ------------------------------------
+    // volume: each wave takes column tiles w, w+NW, ... of the block's
+    // EB*NCOL columns (a q field, a PML flux column or, with PML and
+    // attenuation, a velocity column): B fragment = column 16*tile + l%16 at
+    // node k = 4*ks + l/16
     constexpr int NW = S::NW, CTL = S::CTL, CPW = S::CPW, KS = S::KS, RT = S::RT, NCT = S::NCT;
+    constexpr int MTL = S::MTL, KSL = S::KSL, EPW = S::EPW;
     const int lane = t & 63;
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     R xb[CPW][KS];
@@ -528,7 +510,61 @@ stage_kernel(StageArgs<R> a) {
             xb[c][ks] = (cok && m < NP) ? v : R(0);
         }
     }
-    __syncthreads();   // sQ/sX/sF are dead from here; sV aliases them
+    // LIFT: each wave takes elements w, w+NW, ...: B fragment = face value
+    // l%16 (of NFC) at face node 4*ks + l/16
+    R fb[EPW][KSL];
+#pragma unroll
+    for (int c = 0; c < EPW; ++c) {
+        const int el = w + c * NW;
+        const int f = lane & 15;
+        const bool ok = el < EB && f < NFC;
+        const R *fp = sF + (ok ? el : 0) * NFP * NFCP + (ok ? f : 0);
+#pragma unroll
+        for (int ks = 0; ks < KSL; ++ks) {
+            const R v = fp[(ks * 4 + (lane >> 4)) * NFCP];
+            fb[c][ks] = ok ? v : R(0);
+        }
+    }
+    __syncthreads();   // sQ/sX/sF are dead from here; sV and sL reuse them
+
+    // ---- LIFT on the matrix cores: sL[node][f] = sum_k LIFT(node,k) F(k,f)
+    // (src/timeloopMod.f90:463-465); the face values already carry fscale/2 --
+    {
+        using V4 = typename Mfma<R>::v4;
+        const R *lfr = a.LT + lane;
+#pragma unroll
+        for (int c = 0; c < EPW; ++c) {
+            const int el = w + c * NW;
+            if (el >= EB) break;
+            // LIFT fragments of node tile mt+1 load while mt computes
+            R la[KSL], lb[KSL];
+#pragma unroll
+            for (int ks = 0; ks < KSL; ++ks) la[ks] = lfr[ks * 64];
+#pragma unroll 1
+            for (int mt0 = 0; mt0 < MTL; mt0 += 2) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int mt = mt0 + h;
+                    if (mt >= MTL) break;
+                    R *lc = h == 0 ? la : lb;
+                    R *ln = h == 0 ? lb : la;
+                    const int mn = mt + 1 < MTL ? mt + 1 : mt;
+#pragma unroll
+                    for (int ks = 0; ks < KSL; ++ks) ln[ks] = lfr[(mn * KSL + ks) * 64];
+                    V4 acc = {0, 0, 0, 0};
+#pragma unroll
+                    for (int ks = 0; ks < KSL; ++ks) acc = Mfma<R>::fma(lc[ks], fb[c][ks], acc);
+                    // accumulator element v: node row 4*(l/16)+v (FP32) or
+                    // 4v + l/16 (FP64) of the tile, face value l%16
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int node = mt * 16 + (sizeof(R) == 8 ? 4 * v + (lane >> 4) : 4 * (lane >> 4) + v);
+                        if (node < NP) sL[(el * NP + node) * 16 + (lane & 15)] = acc[v];
+                    }
+                }
+            }
+        }
+    }
 
     // ---- volume term: [Dr;Ds;Dt] x columns on the matrix cores, then the
     // metric of each node (src/timeloopMod.f90:407-420) -----------------------
@@ -630,6 +666,11 @@ stage_kernel(StageArgs<R> a) {
                 qb[f] = valid ? v : R(0);
             }
         }
+    }
+    R lf[NFC];
+    if (node_thr) {
+#pragma unroll
+        for (int f = 0; f < NFC; ++f) lf[f] = sL[t * 16 + f];
     }
     R rhs[9];
     R gv[9];   // velocity gradients: gv[c*3+k] = d v_c / d x_k

@@ -96,6 +96,34 @@ def write_seismograms(outdir: str, seis: np.ndarray, dt: float, recnr=None) -> N
                     f.write(f" {t[i]:.16E} {seis[r, k, i]:.16E}\n")
 
 
+def write_energy(outdir: str, energy: np.ndarray, dt: float) -> None:
+    """``energy``: one row "t E" per step, t = it*dt (src/timeloopMod.f90:650-654)."""
+    os.makedirs(outdir, exist_ok=True)
+    t = np.arange(1, energy.size + 1) * dt
+    with open(os.path.join(outdir, "energy"), "w") as f:
+        for i in range(energy.size):
+            f.write(f" {t[i]:.16E} {energy[i]:.16E}\n")
+
+
+def write_stf(outdir: str, p: problem.Problem) -> None:
+    """``stf%06d`` per source: rows "(it-1)*dt stf" of the source time
+    function before any moment-rate derivative (src/sourceReceiverMod.f90:
+    785-805)."""
+    os.makedirs(outdir, exist_ok=True)
+    for i, src in enumerate(p.sources):
+        base = srcmod.Source(**{**src.__dict__, "kind": 0})
+        val = srcmod.source_time_function(base, p.nt, p.dt)
+        t = (np.arange(1, p.nt + 1, dtype=np.float32) - np.float32(1.0)).astype(np.float64) * p.dt
+        with open(os.path.join(outdir, f"stf{i + 1:06d}"), "w") as f:
+            for k in range(p.nt):
+                f.write(f" {t[k]:.16E} {val[k]:.16E}\n")
+
+
+def read_two_columns(path: str) -> np.ndarray:
+    """Rows "a b" of a seismogram, energy or stf file as an (n, 2) array."""
+    return np.loadtxt(path, ndmin=2)
+
+
 def setup_directory(dirname: str, order: int) -> problem.Problem:
     m = meshmod.read_cubit(os.path.join(dirname, "cubit"))
     par = read_parfile(os.path.join(dirname, "data", "parfile"))
@@ -107,12 +135,17 @@ def setup_directory(dirname: str, order: int) -> problem.Problem:
 
 
 def run(dirname: str, order: int, device: int = 0, compat: int = capi.COMPAT_ALL) -> np.ndarray:
-    """Advance the whole run on the GPU and write out/seismo.* files."""
+    """Advance the whole run on the GPU and write out/seismo.*, out/stf* and,
+    with PML, out/energy as the reference does."""
     p = setup_directory(dirname, order)
     arrs = layout.reference_arrays(p)
+    outdir = os.path.join(dirname, "out")
+    write_stf(outdir, p)
     with capi.Context(arrs, compat=compat, device=device, energy=bool(p.set_pml)) as ctx:
         ctx.reset()
-        ctx.run(1, p.nt)
+        energy = ctx.run(1, p.nt, want_energy=bool(p.set_pml))
         seis = ctx.seismograms()
-    write_seismograms(os.path.join(dirname, "out"), seis, p.dt, p.extra.get("recnr"))
+    write_seismograms(outdir, seis, p.dt, p.extra.get("recnr"))
+    if energy is not None:
+        write_energy(outdir, energy, p.dt)
     return seis

@@ -37,3 +37,29 @@ def test_seismogram_files_round_trip(tmp_path):
     solver.write_seismograms(str(tmp_path), seis, 0.01, recnr=[4, 5, 6])
     back = database.read_seismograms(str(tmp_path), 3, 17, recnr=[4, 5, 6])
     assert np.array_equal(back, seis)
+
+
+def test_stf_file_rows(tmp_path):
+    """stf000001: rows "(it-1)*dt  -ricker(t)" before the moment-rate
+    derivative, checked against the Ricker closed form with the reference's
+    float literals (pi and 1.2 as default REAL)."""
+    p = problem.homogeneous_cube(n=2, order=2, nt=40, source_kind=1)
+    solver.write_stf(str(tmp_path), p)
+    rows = solver.read_two_columns(os.path.join(str(tmp_path), "stf000001"))
+    assert rows.shape == (40, 2)
+    t = (np.arange(40, dtype=np.float32)).astype(np.float64) * p.dt
+    assert np.allclose(rows[:, 0], t, rtol=0, atol=1e-15)
+    s = p.sources[0]
+    pi32 = float(np.float32(np.pi))
+    t0 = float(np.float32(1.2)) / s.f0
+    aval = pi32 * pi32 * s.f0 * s.f0
+    ricker = s.factor * (1.0 - 2.0 * aval * (t - t0) ** 2) * np.exp(-aval * (t - t0) ** 2)
+    assert np.allclose(rows[:, 1], ricker, rtol=1e-13, atol=1e-15)
+
+
+def test_energy_file_round_trip(tmp_path):
+    e = np.linspace(1.0, 2.0, 25) ** 3
+    solver.write_energy(str(tmp_path), e, 0.004)
+    rows = solver.read_two_columns(os.path.join(str(tmp_path), "energy"))
+    assert np.allclose(rows[:, 0], 0.004 * np.arange(1, 26), rtol=1e-15)
+    assert np.array_equal(rows[:, 1], e)
