@@ -139,6 +139,19 @@ int nexd_get_seismograms(nexd_ctx *ctx, double *ux, double *uy, double *uz,
  * displacement (else NEXD_E_STATE for non-NULL ux). */
 int nexd_get_state(nexd_ctx *ctx, double *q, double *ux, double *uy, double *uz);
 
+/* Start from a given state instead of the reference's initial one
+ * (src/timeloopMod.f90:184-195,252-254; the reference itself has no
+ * restart): q (nglob,9), theta (nglob,6,nMB) with
+ * attenuation, fprime/gprime/hprime (nglob,9) with PML, and the displacement
+ * ux,uy,uz (nglob) after `it` completed steps (0 <= it < nt).  NULL leaves a
+ * field as it is (nexd_reset's initial state on a fresh context).  The next
+ * nexd_run must begin at it+1.  Receiver row it+1 is sampled from the given
+ * state; its acceleration (the previous step's last-stage RHS) is not part
+ * of the state and stays at the initial value. */
+int nexd_set_state(nexd_ctx *ctx, int it, const double *q, const double *theta, const double *fprime,
+                   const double *gprime, const double *hprime, const double *ux, const double *uy,
+                   const double *uz);
+
 /* Device time of the last nexd_run in milliseconds (HIP events). */
 int nexd_last_run_ms(nexd_ctx *ctx, double *ms);
 

@@ -610,6 +610,57 @@ int nexd_get_state(nexd_ctx *c, double *q, double *ux, double *uy, double *uz) {
     return NEXD_OK;
 }
 
+int nexd_set_state(nexd_ctx *c, int it, const double *q, const double *theta, const double *fprime,
+                   const double *gprime, const double *hprime, const double *ux, const double *uy,
+                   const double *uz) {
+    if (!c) return fail(c, NEXD_E_ARG, "null context");
+    if (!c->have_ops || !c->have_mesh || !c->have_rec || !c->have_src)
+        return fail(c, NEXD_E_STATE, "upload operators, mesh, sources and receivers first");
+    if (it < 0 || it >= c->cfg.nt) return fail(c, NEXD_E_ARG, "it out of range");
+    if (theta && !c->cfg.attenuation) return fail(c, NEXD_E_ARG, "theta given without attenuation");
+    if ((fprime || gprime || hprime) && !c->cfg.pml) return fail(c, NEXD_E_ARG, "PML fields given without PML");
+    if (!c->initialized) {
+        int rc = nexd_reset(c);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const size_t ng = c->nglob;
+    // device buffer 0 holds the step-start state in the reference's own
+    // column-major layouts: q(nglob,9), theta(nglob,6,nMB), f'(nglob,9)...
+    if (q) HIPCHK(c, hipMemcpy(c->q[0], q, sizeof(double) * 9 * ng, hipMemcpyHostToDevice));
+    if (theta) HIPCHK(c, hipMemcpy(c->th[0], theta, sizeof(double) * 18 * ng, hipMemcpyHostToDevice));
+    const double *pf[3] = {fprime, gprime, hprime};
+    for (int k = 0; k < 3; ++k)
+        if (pf[k]) HIPCHK(c, hipMemcpy(c->paux[0] + (size_t)k * 9 * ng, pf[k], sizeof(double) * 9 * ng, hipMemcpyHostToDevice));
+    const double *u[3] = {ux, uy, uz};
+    for (int k = 0; k < 3; ++k) {
+        if (!u[k]) continue;
+        if (c->u_full) HIPCHK(c, hipMemcpy(c->u_full + k * ng, u[k], sizeof(double) * ng, hipMemcpyHostToDevice));
+    }
+    if (c->nslot && (ux || uy || uz)) {
+        // receiver-element displacement slots from the given fields
+        std::vector<int> slot_of_elem(c->cfg.nelem, -1);
+        HIPCHK(c, hipMemcpy(slot_of_elem.data(), c->rec_slot, sizeof(int) * c->cfg.nelem, hipMemcpyDeviceToHost));
+        std::vector<double> us((size_t)3 * c->nslot * c->np);
+        HIPCHK(c, hipMemcpy(us.data(), c->u_slot, sizeof(double) * us.size(), hipMemcpyDeviceToHost));
+        for (int e = 0; e < c->cfg.nelem; ++e) {
+            const int sl = slot_of_elem[e];
+            if (sl < 0) continue;
+            for (int k = 0; k < 3; ++k)
+                if (u[k])
+                    for (int j = 0; j < c->np; ++j) us[((size_t)sl * 3 + k) * c->np + j] = u[k][(size_t)e * c->np + j];
+        }
+        HIPCHK(c, hipMemcpy(c->u_slot, us.data(), sizeof(double) * us.size(), hipMemcpyHostToDevice));
+    }
+    HIPCHK(c, hipMemcpy(c->d_it, &it, sizeof(int), hipMemcpyHostToDevice));
+    c->step = it;
+    // receiver row it+1 is the state before step it+1: the state just set
+    dispatch_step_end(c, 0, 0);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipGetLastError());
+    return NEXD_OK;
+}
+
 int nexd_last_run_ms(nexd_ctx *c, double *ms) {
     if (!c || !ms) return fail(c, NEXD_E_ARG, "null argument");
     *ms = c->last_ms;

@@ -37,6 +37,7 @@ struct Dims {
 };
 
 constexpr int NMB = 3;
+constexpr unsigned kXcds = 8;   // accelerator complex dies of an MI355X
 
 template <typename R>
 struct StageArgs {
@@ -347,10 +348,17 @@ stage_kernel(StageArgs<R> a) {
     const int t = threadIdx.x;
     const size_t ng = (size_t)a.nglob;
     const int it_step = *a.d_it;
-    // shots vary fastest across blocks, so the blocks of one element group run
-    // together and share its geometry through L2
-    const int shot = (int)(blockIdx.x % (unsigned)a.nshot);
-    const int eblk = (int)(blockIdx.x / (unsigned)a.nshot);
+    // The hardware deals consecutive workgroups round-robin to the XCDs, each
+    // with its own L2.  The logical block index gives each XCD one
+    // contiguous range instead (a bijection for any grid), so neighbouring
+    // element groups, whose traces the face phase gathers, share an L2.
+    // Shots vary fastest within a range, so the blocks of one element group
+    // run together and share its geometry.
+    const unsigned nb = gridDim.x, xcd = blockIdx.x % kXcds, rank = blockIdx.x / kXcds;
+    const unsigned per = nb / kXcds, extra = nb % kXcds;
+    const unsigned lb = xcd * per + (xcd < extra ? xcd : extra) + rank;
+    const int shot = (int)(lb % (unsigned)a.nshot);
+    const int eblk = (int)(lb / (unsigned)a.nshot);
     const R *qin = a.qin + (size_t)shot * 9 * ng;
     const R *qbase = a.qbase + (size_t)shot * 9 * ng;
     R *qout = a.qout + (size_t)shot * 9 * ng;
@@ -364,79 +372,76 @@ stage_kernel(StageArgs<R> a) {
     const int e = valid ? a.elist[idx] : 0;
     const size_t g = (size_t)e * NP + j;
 
+    // One straight-line block per node thread, so the scheduler can issue
+    // the loads ahead of the stores: q, metric and material, then (PML) the
+    // coefficients and the three directions' auxiliaries, last-element fluxes
+    // and RK bases, each direction loaded before the previous one is
+    // computed.  No load is guarded by a branch: padding threads read element
+    // 0, and in stage 1 the RK base buffer is the input buffer (the value is
+    // not used).
     R qs[9];
     R ef0 = 0, ef1 = 0, ef2 = 0, ef3 = 0, rho = 1;
     if (node_thr) {
+        R qv[9], mv[9];
+#pragma unroll
+        for (int f = 0; f < 9; ++f) qv[f] = qin[f * ng + g];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) mv[k] = a.metric[k * ng + g];
+        const R *mc = matc + (size_t)e * 8;
+        const R m2 = mc[2], m3 = mc[3], m4 = mc[4];
+        // PML inputs (unused and not loaded without PML)
+        const R *pmlc = a.pmlc + (size_t)shot * a.pmlc_stride_ng * ng;
+        const R *pin = a.pin + (size_t)shot * 27 * ng;
+        const R *pbase = a.pbase + (size_t)shot * 27 * ng;
+        R *pout = a.pout + (size_t)shot * 27 * ng;
+        const R *lastf = a.last_flux + (size_t)shot * 27 * NP;
+        R pc[9], cur[3][9], use[3][9], bse[3][9];
+        auto load_dir = [&](int d) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                cur[d][i] = pin[(d * 9 + i) * ng + g];
+                use[d][i] = lastf[(d * 9 + i) * NP + j];
+                bse[d][i] = pbase[(d * 9 + i) * ng + g];
+            }
+        };
+        if constexpr (PML) {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) pc[k] = pmlc[k * ng + g];
+            load_dir(0);
+        }
 #pragma unroll
         for (int f = 0; f < 9; ++f) {
-            const R v = qin[f * ng + g];
-            qs[f] = valid ? v : R(0);
+            qs[f] = valid ? qv[f] : R(0);
             sQ[f * NT + t] = qs[f];
         }
 #pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            const R v = a.metric[k * ng + g];
-            sM[k * NT + t] = valid ? v : R(0);
-        }
-        const R *mc = matc + (size_t)e * 8;
-        const R m2 = mc[2], m3 = mc[3], m4 = mc[4];
+        for (int k = 0; k < 9; ++k) sM[k * NT + t] = valid ? mv[k] : R(0);
         if (valid) {
             rho = m2;
             ef0 = -(m3 + 2 * m4); ef1 = -m3; ef2 = -m4; ef3 = R(-1.0) / rho;
         }
-    }
-    if constexpr (PML) {
-        if (node_thr) {
+        if constexpr (PML) {
             // modified fluxes (src/timeloopMod.f90:399-405) and the ADE update
             // of f',g',h' (src/rungekuttaMod.f90:52-63); the update needs only
             // stage-start values, so it is done here and the auxiliaries die
             R Fm[9], Gm[9], Hm[9];
             elastic_flux_node(qs, ef0, ef1, ef2, ef3, Fm, Gm, Hm);
-            const R *pmlc = a.pmlc + (size_t)shot * a.pmlc_stride_ng * ng;
-            const R *pin = a.pin + (size_t)shot * 27 * ng;
-            const R *pbase = a.pbase + (size_t)shot * 27 * ng;
-            R *pout = a.pout + (size_t)shot * 27 * ng;
-            const R *lastf = a.last_flux + (size_t)shot * 27 * NP;
-            R kk[3], al[3], dk[3];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                const R v0 = pmlc[d * ng + g], v1 = pmlc[(3 + d) * ng + g], v2 = pmlc[(6 + d) * ng + g];
-                al[d] = valid ? v0 : R(0); dk[d] = valid ? v1 : R(0); kk[d] = valid ? v2 : R(1);
-            }
             int col = 0;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
+                if (d + 1 < 3) load_dir(d + 1);
+                const R al = valid ? pc[d] : R(0), dk = valid ? pc[3 + d] : R(0), kk = valid ? pc[6 + d] : R(1);
                 R *FF = d == 0 ? Fm : (d == 1 ? Gm : Hm);
                 const int skip = d == 0 ? 4 : (d == 1 ? 5 : 3);
-                // one direction's auxiliaries, last-element fluxes and RK
-                // base are loaded together before any store
-                R cur[9], use[9], base[9];
 #pragma unroll
                 for (int i = 0; i < 9; ++i) {
-                    const R v = pin[(d * 9 + i) * ng + g];
-                    cur[i] = valid ? v : R(0);
-                }
-                if (a.pml_compat) {
-#pragma unroll
-                    for (int i = 0; i < 9; ++i) use[i] = lastf[(d * 9 + i) * NP + j];
-                }
-                if (a.irk != 1) {
-#pragma unroll
-                    for (int i = 0; i < 9; ++i) {
-                        const R v = pbase[(d * 9 + i) * ng + g];
-                        base[i] = valid ? v : R(0);
-                    }
-                } else {
-#pragma unroll
-                    for (int i = 0; i < 9; ++i) base[i] = cur[i];
-                }
-#pragma unroll
-                for (int i = 0; i < 9; ++i) {
-                    const R mod = (cur[i] + FF[i]) / kk[d];
+                    const R c0 = valid ? cur[d][i] : R(0);
+                    const R mod = (c0 + FF[i]) / kk;
                     if (i != skip) sX[(col++) * NT + t] = mod;
-                    const R u = a.pml_compat ? use[i] : mod;
-                    const R dadt = -al[d] * cur[i] - dk[d] * (cur[i] + u);
-                    const R v = rk_combine<R>(a.scheme, a.irk, cur[i], base[i], dadt, a.dt, a.w13, a.w23);
+                    const R u = a.pml_compat ? use[d][i] : mod;
+                    const R dadt = -al * c0 - dk * (c0 + u);
+                    const R b0 = valid ? bse[d][i] : R(0);
+                    const R v = rk_combine<R>(a.scheme, a.irk, c0, b0, dadt, a.dt, a.w13, a.w23);
                     if (valid) pout[(d * 9 + i) * ng + g] = v;
                 }
             }
@@ -483,6 +488,34 @@ stage_kernel(StageArgs<R> a) {
         for (int f = NFC; f < NFCP; ++f) dst[f] = 0;
     }
     __syncthreads();
+
+    // The memory variables and anelastic coefficients of the assembly are
+    // loaded in one batch; FP32 blocks issue that batch before the
+    // matrix-core phase, whose ~20k cycles then hide the round trip (the
+    // barriers wait for LDS traffic only).  FP64 blocks, and FP32 blocks
+    // outside the PML, have no registers to spare and load it at the
+    // assembly, as all do for the RK bases.
+    constexpr bool kEarly = sizeof(R) == 4 && PML;
+    R th[ATT ? 18 : 1], avc[ATT ? 9 : 1];
+    auto load_memory_variables = [&]() {
+        if constexpr (ATT) {
+            if (node_thr) {
+                const R *thin = a.thin + (size_t)shot * 18 * ng;
+                const R *av = a.anel + (size_t)shot * a.anel_stride + (size_t)e * NMB * 3;
+#pragma unroll
+                for (int c = 0; c < 18; ++c) {
+                    const R v = thin[c * ng + g];
+                    th[c] = valid ? v : R(0);
+                }
+#pragma unroll
+                for (int c = 0; c < 9; ++c) {
+                    const R v = av[c];
+                    avc[c] = valid ? v : R(0);
+                }
+            }
+        }
+    };
+    if constexpr (kEarly) load_memory_variables();
 
     // ---- operand fragments into registers ------------------------------------
     // volume: each wave takes column tiles w, w+NW, ... of the block's
@@ -627,19 +660,11 @@ stage_kernel(StageArgs<R> a) {
     __syncthreads();
 
     // ---- assemble the right-hand side (node threads) -----------------------
-    // the memory variables, anelastic coefficients and RK bases this phase
-    // needs are loaded first, in one batch, before any of its stores
-    R th[ATT ? 18 : 1], thb[ATT ? 18 : 1], avc[ATT ? 9 : 1], qb[9];
+    if constexpr (!kEarly) load_memory_variables();
+    R thb[ATT ? 18 : 1], qb[9];
     if (node_thr) {
         if constexpr (ATT) {
-            const R *thin = a.thin + (size_t)shot * 18 * ng;
             const R *thbase = a.thbase + (size_t)shot * 18 * ng;
-            const R *av = a.anel + (size_t)shot * a.anel_stride + (size_t)e * NMB * 3;
-#pragma unroll
-            for (int c = 0; c < 18; ++c) {
-                const R v = thin[c * ng + g];
-                th[c] = valid ? v : R(0);
-            }
             if (a.irk == 1) {
 #pragma unroll
                 for (int c = 0; c < 18; ++c) thb[c] = th[c];
@@ -649,11 +674,6 @@ stage_kernel(StageArgs<R> a) {
                     const R v = thbase[c * ng + g];
                     thb[c] = valid ? v : R(0);
                 }
-            }
-#pragma unroll
-            for (int c = 0; c < 9; ++c) {
-                const R v = av[c];
-                avc[c] = valid ? v : R(0);
             }
         }
         if (a.irk == 1) {

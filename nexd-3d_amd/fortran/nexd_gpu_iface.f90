@@ -107,6 +107,20 @@ module nexd_gpu_iface
             real(c_double), intent(out) :: ux(*), uy(*), uz(*), vx(*), vy(*), vz(*), ax(*), ay(*), az(*)
         end function
 
+        ! arrays passed as c_loc(x) or c_null_ptr (NULL: not wanted / keep)
+        integer(c_int) function nexd_get_state(ctx, q, ux, uy, uz) bind(C, name="nexd_get_state")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, q, ux, uy, uz
+        end function
+
+        integer(c_int) function nexd_set_state(ctx, it, q, theta, fprime, gprime, hprime, ux, uy, uz) &
+                bind(C, name="nexd_set_state")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it
+            type(c_ptr), value :: q, theta, fprime, gprime, hprime, ux, uy, uz
+        end function
+
         subroutine nexd_destroy(ctx) bind(C, name="nexd_destroy")
             import :: c_ptr
             type(c_ptr), value :: ctx

@@ -22,6 +22,7 @@ COMPAT_ALL = 3
 EXPORTS = ["nexd_create", "nexd_upload_operators", "nexd_upload_mesh", "nexd_upload_material",
            "nexd_upload_anelastic", "nexd_upload_pml", "nexd_upload_sources", "nexd_upload_receivers",
            "nexd_reset", "nexd_run", "nexd_set_pml_active", "nexd_get_seismograms", "nexd_get_state",
+           "nexd_set_state",
            "nexd_last_run_ms", "nexd_time_stage_kernel", "nexd_destroy", "nexd_last_error"]
 
 
@@ -139,11 +140,28 @@ class Context:
         # each output is Fortran (nrec,nt): element (r,it) at it*nrec + r
         return np.stack([o.reshape(nt, nrec).T for o in outs], axis=1)
 
-    def state(self) -> np.ndarray:
+    def state(self, displacement: bool = False):
+        """q as (9, nglob); with displacement=True also u as (3, nglob)
+        (the context must track displacement)."""
         ng = self.arrs["nelem"] * self.arrs["np"]
         q = np.zeros(9 * ng)
-        self._chk(self.lib.nexd_get_state(self.h, _d(q), None, None, None), "get_state")
-        return q.reshape(9, ng)
+        if not displacement:
+            self._chk(self.lib.nexd_get_state(self.h, _d(q), None, None, None), "get_state")
+            return q.reshape(9, ng)
+        u = np.zeros((3, ng))
+        self._chk(self.lib.nexd_get_state(self.h, _d(q), _d(u[0]), _d(u[1]), _d(u[2])), "get_state")
+        return q.reshape(9, ng), u
+
+    def set_state(self, it: int, q=None, theta=None, fprime=None, gprime=None, hprime=None, u=None):
+        """Continue from a state after `it` steps (layouts as nexd_set_state)."""
+        def arr(x):
+            return None if x is None else np.ascontiguousarray(x, dtype=np.float64)
+        q, theta, fprime, gprime, hprime = map(arr, (q, theta, fprime, gprime, hprime))
+        ux = uy = uz = None
+        if u is not None:
+            ux, uy, uz = (np.ascontiguousarray(u[k], dtype=np.float64) for k in range(3))
+        self._keep = (q, theta, fprime, gprime, hprime, ux, uy, uz)
+        self._chk(self.lib.nexd_set_state(self.h, it, *[_d(x) for x in self._keep]), "set_state")
 
     def last_run_ms(self) -> float:
         ms = ctypes.c_double()

@@ -117,3 +117,27 @@ def test_error_codes():
     bad["ibn"].reshape(-1)[np.argmax(bad["ibn"].reshape(-1) > 0)] = 10 ** 8
     with pytest.raises(capi.NexdError, match="ibn out of range"):
         capi.Context(bad)
+
+
+def test_restart_from_state():
+    """nexd_set_state: 30 steps, state out, continue in a fresh context; the
+    seismograms from the restart row on equal one continuous run bitwise
+    (the restart row's acceleration is not part of the state)."""
+    from nexd_amd import problem
+    p = problem.homogeneous_cube(n=3, order=3, nt=80, source_kind=1)
+    arrs = layout.reference_arrays(p)
+    print(f"[gpu restart] nelem={p.nelem} nt={p.nt}", flush=True)
+    with capi.Context(arrs, track_displacement=True) as full:
+        full.reset()
+        full.run(1, 80)
+        ref = full.seismograms()
+    with capi.Context(arrs, track_displacement=True) as first:
+        first.reset()
+        first.run(1, 30)
+        q, u = first.state(displacement=True)
+    with capi.Context(arrs, track_displacement=True) as second:
+        second.set_state(30, q=q, u=u)
+        second.run(31, 80)
+        got = second.seismograms()
+    assert np.array_equal(got[:, :6, 30:], ref[:, :6, 30:])
+    assert np.array_equal(got[:, 6:, 31:], ref[:, 6:, 31:])
