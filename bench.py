@@ -170,9 +170,9 @@ def main():
     ap.add_argument("--nz", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
-    ap.add_argument("--accuracy-nt", type=int, default=0, metavar="NT",
-                    help="also run shot 0 for NT steps in FP64 and in FP32 (plain and compensated) and report "
-                         "the seismogram error of each FP32 form against FP64")
+    ap.add_argument("--accuracy-nt", type=int, default=4000, metavar="NT",
+                    help="also run shot 0 for NT steps (default: the workload's 4000) in FP64 and in FP32 (plain "
+                         "and compensated) and report the seismogram error of each FP32 form against FP64; 0 skips")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (calibrates HBM byte counters)")
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
