@@ -159,6 +159,48 @@ def accuracy_run(args, pml_cells):
     return out
 
 
+def uneven_main(args):
+    """Uneven batch: K independent instances of different size and order
+    (layered elastic, one point force each), FP32, nt = warmup + steps.  The
+    group (one parent graph per step) is timed against the same instances
+    run one after another."""
+    from nexd_amd import batch, capi
+    t0 = time.time()
+    inst = batch.uneven_instances(args.uneven, nt=args.warmup + args.steps)
+    dof = sum(sb.base.nelem * sb.base.ref.np_ * 9 for sb in inst)
+    log(f"[bench] uneven set-up {time.time() - t0:.1f}s: {args.uneven} instances, "
+        f"{sum(sb.base.nelem for sb in inst)} tets, orders {sorted(set(sb.base.order for sb in inst))}")
+    alone_ms = 0.0
+    for sb in inst:
+        with capi.BatchContext(sb, precision=capi.NEXD_FP32) as b:
+            b.reset()
+            if args.warmup:
+                b.run(1, args.warmup)
+            b.run(args.warmup + 1, args.warmup + args.steps)
+            alone_ms += b.last_run_ms()
+    members = [capi.BatchContext(sb, precision=capi.NEXD_FP32) for sb in inst]
+    for m in members:
+        m.reset()
+    with capi.Group(members) as g:
+        if args.warmup:
+            g.run(1, args.warmup)
+        g.run(args.warmup + 1, args.warmup + args.steps)
+        group_ms = g.last_run_ms()
+    for m in members:
+        m.close()
+    value = dof * args.steps / (group_ms * 1e-3)
+    log(f"[bench] uneven: group {group_ms / args.steps:.3f} ms/step, one by one {alone_ms / args.steps:.3f} ms/step")
+    print(json.dumps({
+        "metric": "DOF-updates/s on 1 GPU, uneven batch (one graph per step vs one by one)",
+        "value": value, "unit": "DOF-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": group_ms / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{args.uneven} instances, 6n^3 tets with n uniform in 10..30, N uniform in "
+                               f"{{2,3,4,5}}, 2-4 elastic layers, one point force each, RK2",
+                   "dof_per_step": dof, "one_by_one_dof_updates_per_s": dof * args.steps / (alone_ms * 1e-3),
+                   "group_over_one_by_one": alone_ms / group_ms}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,6 +217,9 @@ def main():
                          "and compensated) and report the seismogram error of each FP32 form against FP64; 0 skips")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (calibrates HBM byte counters)")
+    ap.add_argument("--uneven", type=int, default=0, metavar="K",
+                    help="instead: K uneven instances (6n^3 tets, n in 10..30, N in 2..5, FP32) advanced as one "
+                         "group vs one by one")
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
                     help="only summarise rocprofv3 counter runs into per-launch HBM bytes (JSON on stdout)")
     args = ap.parse_args()
@@ -186,6 +231,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.uneven:
+        uneven_main(args)
+        return
     if args.copy_probe:
         import torch
         n = args.copy_probe * (1 << 20) // 4

@@ -236,6 +236,25 @@ int nexd_batch_time_stage_kernel(nexd_batch *b, int list, int nrep, double *avg_
 void nexd_batch_destroy(nexd_batch *b);
 const char *nexd_batch_last_error(const nexd_batch *b);
 
+/* ==========================================================================
+ * Groups of batches (no reference counterpart): independent instances of
+ * different meshes, sizes, orders and precisions advanced together.  Each
+ * member's step graph becomes a child node of one parent HIP graph without
+ * dependencies between members, launched once per step, so small instances
+ * overlap on the device instead of running one after another.
+ * Members must share nt and the device and stand at the same step; they
+ * keep their own state, seismograms and getters.  Do not upload to a member
+ * or run it on its own while it is in a group; destroy the group first.
+ * ========================================================================== */
+typedef struct nexd_group nexd_group;
+
+int nexd_group_create(nexd_batch *const *members, int n, nexd_group **out);
+/* advance every member from step it_begin to it_end (1-based, inclusive) */
+int nexd_group_run(nexd_group *g, int it_begin, int it_end);
+int nexd_group_last_run_ms(nexd_group *g, double *ms);
+void nexd_group_destroy(nexd_group *g);
+const char *nexd_group_last_error(const nexd_group *g);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,6 +22,9 @@ thread_local std::string g_batch_create_error;
 
 struct BatchBase {
     virtual ~BatchBase() = default;
+    virtual int group_prepare(hipGraph_t *out) = 0;
+    virtual int next_step() const = 0;
+    virtual void set_steps_done(int s) = 0;
     nexd_batch_config cfg{};
     std::string err;
 };
@@ -391,6 +394,29 @@ struct Batch : BatchBase {
         return 0;
     }
 
+    // ---- group membership: the graph of one step, without instantiating it,
+    // for a parent graph that advances several batches together
+    int group_prepare(hipGraph_t *out) override {
+        if (!ready()) return fail(NEXD_E_STATE, "upload operators, mesh, materials, PML, sources and receivers first");
+        if (!initialized) {
+            int rc = reset();
+            if (rc) return rc;
+        }
+        if (hipSetDevice(cfg.device) != hipSuccess) return fail(NEXD_E_HIP, "hipSetDevice failed");
+        if (step == 0) {
+            step_end(0, 0);   // receiver row 1: the initial state
+            if (hipStreamSynchronize(stream) != hipSuccess) return fail(NEXD_E_HIP, "hipStreamSynchronize failed");
+        }
+        if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return fail(NEXD_E_HIP, "hipStreamBeginCapture failed");
+        enqueue_step();
+        if (hipStreamEndCapture(stream, out) != hipSuccess) return fail(NEXD_E_HIP, "hipStreamEndCapture failed");
+        return 0;
+    }
+
+    int next_step() const override { return step + 1; }
+    void set_steps_done(int s) override { step = s; }
+
     int time_stage(int list, int nrep, double *avg_ms, int *nel) {
         if (!initialized) return fail(NEXD_E_STATE, "run or reset before timing");
         if (nrk == 1) return fail(NEXD_E_UNSUPPORTED, "timing needs a multi-stage scheme");
@@ -607,7 +633,108 @@ int get_seis(Batch<R> *b, int shot, double *const outs[9]) {
 
 }  // namespace
 
+// A group of batches advanced together: each member's step graph becomes a
+// child node of one parent graph, with no dependencies between members, so
+// their launches overlap on the device; one graph launch per step.
+struct nexd_group {
+    std::vector<BatchBase *> members;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int step = 0, nt = 0, device = 0;
+    float last_ms = 0.f;
+    std::string err;
+    int fail(int code, const std::string &m) { err = m; return code; }
+    ~nexd_group() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+static std::string g_group_create_error;
+
 extern "C" {
+
+int nexd_group_create(nexd_batch *const *members, int n, nexd_group **out) {
+    if (!members || n <= 0 || !out) { g_group_create_error = "null argument or empty group"; return NEXD_E_ARG; }
+    *out = nullptr;
+    std::unique_ptr<nexd_group> g(new nexd_group());
+    for (int i = 0; i < n; ++i) {
+        if (!members[i]) { g_group_create_error = "null member"; return NEXD_E_ARG; }
+        for (int k = 0; k < i; ++k)
+            if (members[k] == members[i]) { g_group_create_error = "a batch appears twice"; return NEXD_E_ARG; }
+        BatchBase *m = base(members[i]);
+        if (i == 0) { g->nt = m->cfg.nt; g->device = m->cfg.device; g->step = m->next_step() - 1; }
+        if (m->cfg.nt != g->nt || m->cfg.device != g->device || m->next_step() - 1 != g->step) {
+            g_group_create_error = "members must share nt, device and the current step";
+            return NEXD_E_ARG;
+        }
+        g->members.push_back(m);
+    }
+    if (hipSetDevice(g->device) != hipSuccess || hipStreamCreate(&g->stream) != hipSuccess ||
+        hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess) {
+        g_group_create_error = "HIP stream/event creation failed";
+        return NEXD_E_HIP;
+    }
+    hipGraph_t parent = nullptr;
+    if (hipGraphCreate(&parent, 0) != hipSuccess) { g_group_create_error = "hipGraphCreate failed"; return NEXD_E_HIP; }
+    for (BatchBase *m : g->members) {
+        hipGraph_t child = nullptr;
+        int rc = m->group_prepare(&child);
+        if (rc) { g_group_create_error = "member: " + m->err; (void)hipGraphDestroy(parent); return rc; }
+        hipGraphNode_t node;
+        hipError_t e = hipGraphAddChildGraphNode(&node, parent, nullptr, 0, child);
+        (void)hipGraphDestroy(child);
+        if (e != hipSuccess) {
+            g_group_create_error = std::string("hipGraphAddChildGraphNode: ") + hipGetErrorString(e);
+            (void)hipGraphDestroy(parent);
+            return NEXD_E_HIP;
+        }
+    }
+    hipError_t e = hipGraphInstantiate(&g->exec, parent, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(parent);
+    if (e != hipSuccess) {
+        g->exec = nullptr;
+        g_group_create_error = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
+        return NEXD_E_HIP;
+    }
+    *out = g.release();
+    return NEXD_OK;
+}
+
+int nexd_group_run(nexd_group *g, int it_begin, int it_end) {
+    if (!g) return NEXD_E_ARG;
+    if (it_begin != g->step + 1) return g->fail(NEXD_E_ARG, "it_begin must be the next step");
+    if (it_end < it_begin || it_end > g->nt) return g->fail(NEXD_E_ARG, "it_end out of range");
+    if (hipSetDevice(g->device) != hipSuccess) return g->fail(NEXD_E_HIP, "hipSetDevice failed");
+    for (BatchBase *m : g->members)
+        if (m->next_step() != it_begin) return g->fail(NEXD_E_STATE, "a member was advanced outside the group");
+    if (hipEventRecord(g->ev0, g->stream) != hipSuccess) return g->fail(NEXD_E_HIP, "hipEventRecord failed");
+    for (int it = it_begin; it <= it_end; ++it)
+        if (hipGraphLaunch(g->exec, g->stream) != hipSuccess) return g->fail(NEXD_E_HIP, "hipGraphLaunch failed");
+    if (hipEventRecord(g->ev1, g->stream) != hipSuccess) return g->fail(NEXD_E_HIP, "hipEventRecord failed");
+    hipError_t e = hipStreamSynchronize(g->stream);
+    if (e != hipSuccess) return g->fail(NEXD_E_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+    e = hipGetLastError();
+    if (e != hipSuccess) return g->fail(NEXD_E_HIP, std::string("kernel: ") + hipGetErrorString(e));
+    (void)hipEventElapsedTime(&g->last_ms, g->ev0, g->ev1);
+    for (BatchBase *m : g->members) m->set_steps_done(it_end);
+    g->step = it_end;
+    return NEXD_OK;
+}
+
+int nexd_group_last_run_ms(nexd_group *g, double *ms) {
+    if (!g || !ms) return NEXD_E_ARG;
+    *ms = g->last_ms;
+    return NEXD_OK;
+}
+
+void nexd_group_destroy(nexd_group *g) { delete g; }
+
+const char *nexd_group_last_error(const nexd_group *g) { return g ? g->err.c_str() : g_group_create_error.c_str(); }
 
 int nexd_batch_create(const nexd_batch_config *cfg, nexd_batch **out) {
     if (!cfg || !out) { g_batch_create_error = "null argument"; return NEXD_E_ARG; }

@@ -125,3 +125,40 @@ def layered_shots(nshot: int, seed: int = 20, full_source_arrays: bool = False, 
         loc = srcmod.locate([xyz], base.mesh, base.geom, base.order)
         shots.append(dataclasses.replace(base, sources=[src], src=None, extra={"loc": loc}))
     return _finish(base, shots, shared=True, full_source_arrays=full_source_arrays)
+
+
+def uneven_instances(count: int, seed: int = 4000, n_lo: int = 10, n_hi: int = 30, orders=(2, 3, 4, 5),
+                     nt: int = 200, hsize: float = 500.0, nrec: int = 8, f0: float = 1.0,
+                     timeint: int = 2) -> list:
+    """Independent instances of different size and order (one shot each):
+    n uniform in [n_lo, n_hi] hexes per side (6 n^3 tets), N uniform in
+    ``orders``, 2-4 horizontal elastic layers with vp uniform in 2500-6500
+    m/s, vs = vp/sqrt(3), rho 2700, a free top and absorbing sides and
+    bottom, one vertical point force (Ricker f0) at a random position in the
+    central half of the box and ``nrec`` receivers on a surface line; seed
+    ``seed + k`` for instance k.  Each instance has its own CFL step."""
+    out = []
+    for k in range(count):
+        rng = np.random.default_rng(seed + k)
+        n = int(rng.integers(n_lo, n_hi + 1))
+        order = int(rng.choice(orders))
+        ext = n * hsize
+        nl = int(rng.integers(2, 5))
+        tops = np.sort(rng.uniform(0.1, 0.9, nl - 1))[::-1] * ext   # interface heights, top layer first
+        vps = rng.uniform(2500.0, 6500.0, nl)
+
+        def material(x, y, z, tops=tops, vps=vps):
+            layer = np.zeros_like(z, dtype=np.int64)
+            for i, h in enumerate(tops):
+                layer = np.where(z < h, i + 1, layer)
+            vp = vps[layer]
+            return vp, vp / math.sqrt(3.0), np.full_like(vp, 2700.0), np.full_like(vp, 9000.0), np.full_like(vp, 9000.0)
+
+        m = meshmod.brick_mesh(n, n, n, (ext,) * 3, material=material)
+        xyz = tuple(float(v) for v in rng.uniform(0.25 * ext, 0.75 * ext, 3))
+        src = srcmod.Source(xyz=xyz, kind=0, stf=2, f0=f0, factor=1e10, angle_force=(0.0, 0.0, 1.0))
+        recs = [((i + 0.5) * ext / nrec, ext / 2, ext) for i in range(nrec)]
+        p = problem.setup(m, order, dict(timeint=timeint, nt=nt, cfl=0.25), [src], recs,
+                          name=f"uneven{k}_n{n}_N{order}", full_source_arrays=False)
+        out.append(ShotBatch(p, [p], True))
+    return out

@@ -203,6 +203,8 @@ BATCH_EXPORTS = ["nexd_batch_create", "nexd_batch_upload_operators", "nexd_batch
                  "nexd_batch_upload_sources", "nexd_batch_upload_receivers", "nexd_batch_reset", "nexd_batch_run",
                  "nexd_batch_get_seismograms", "nexd_batch_last_run_ms", "nexd_batch_time_stage_kernel",
                  "nexd_batch_destroy", "nexd_batch_last_error"]
+GROUP_EXPORTS = ["nexd_group_create", "nexd_group_run", "nexd_group_last_run_ms", "nexd_group_destroy",
+                 "nexd_group_last_error"]
 
 
 class NexdBatchConfig(ctypes.Structure):
@@ -303,6 +305,61 @@ class BatchContext:
     def close(self):
         if getattr(self, "h", None):
             self.lib.nexd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Group:
+    """Independent batches advanced together, one parent HIP graph per step
+    (nexd_group_*).  The members keep their own state and getters."""
+
+    def __init__(self, members):
+        self.lib = _load_batch()
+        L = self.lib
+        if not getattr(L, "_group_ready", False):
+            L.nexd_group_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_void_p)]
+            L.nexd_group_run.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+            L.nexd_group_last_run_ms.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+            L.nexd_group_destroy.restype = None
+            L.nexd_group_destroy.argtypes = [ctypes.c_void_p]
+            L.nexd_group_last_error.restype = ctypes.c_char_p
+            L.nexd_group_last_error.argtypes = [ctypes.c_void_p]
+            for n in ("nexd_group_create", "nexd_group_run", "nexd_group_last_run_ms"):
+                getattr(L, n).restype = ctypes.c_int
+            L._group_ready = True
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m.h for m in self.members])
+        h = ctypes.c_void_p()
+        rc = L.nexd_group_create(arr, len(self.members), ctypes.byref(h))
+        if rc != 0:
+            raise NexdError(f"nexd_group_create: {rc}: {L.nexd_group_last_error(None).decode()}")
+        self.h = h
+
+    def run(self, it_begin: int, it_end: int):
+        rc = self.lib.nexd_group_run(self.h, it_begin, it_end)
+        if rc != 0:
+            raise NexdError(f"nexd_group_run: {rc}: {self.lib.nexd_group_last_error(self.h).decode()}")
+
+    def last_run_ms(self) -> float:
+        ms = ctypes.c_double()
+        self.lib.nexd_group_last_run_ms(self.h, ctypes.byref(ms))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nexd_group_destroy(self.h)
             self.h = None
 
     def __del__(self):

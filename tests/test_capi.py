@@ -38,3 +38,14 @@ def test_library_exports_every_declared_symbol():
 def test_last_error_without_context_is_a_string():
     lib = capi.load()
     assert isinstance(lib.nexd_last_error(None), bytes)
+
+
+def test_group_create_rejects_empty_group_without_a_gpu_call():
+    lib = capi._load_batch()
+    h = ctypes.c_void_p()
+    arr = (ctypes.c_void_p * 1)()
+    rc = lib.nexd_group_create(arr, 0, ctypes.byref(h))
+    assert rc == -1
+    lib.nexd_group_last_error.restype = ctypes.c_char_p
+    lib.nexd_group_last_error.argtypes = [ctypes.c_void_p]
+    assert b"empty" in lib.nexd_group_last_error(None)
