@@ -96,3 +96,31 @@ def test_fp32_compensated_error_vs_fp64(which, cube_shots, layered_shots):
         err, err0 = rel_l2(got[k], ref), rel_l2(plain[k], ref)
         print(f"  {which} shot {k}: FP32 compensated {err.max():.3e}, plain FP32 {err0.max():.3e}", flush=True)
         assert err.max() <= 1e-4
+
+
+@pytest.mark.parametrize("timeint", [1, 3])
+def test_fp64_batch_equals_single_shot_other_schemes(timeint):
+    """Euler (buffer copy after the stage) and RK3 (three buffers, float
+    weights) through the batch path against the single-shot path."""
+    nt = 40 if timeint == 1 else 150
+    # (40 Euler steps at this CFL grow slowly; both paths do the same arithmetic)
+    sb = batch.multishot_cube(2, n=3, order=3, nt=nt, timeint=timeint, full_source_arrays=True)
+    print(f"[gpu batch] timeint={timeint} shots={sb.nshot}", flush=True)
+    got = batch_run(sb, capi.NEXD_FP64)
+    for k, s in enumerate(sb.shots):
+        ref = fp64_single(s)
+        assert np.max(rel_l2(got[k], ref)) <= 1e-10
+
+
+def test_fp32_compensated_rk3():
+    """The RK3 branch of the compensated stage sum (weights 3/4, 1/4 and the
+    float-rounded 1/3, 2/3)."""
+    sb = batch.multishot_cube(2, n=3, order=3, nt=150, timeint=3, full_source_arrays=True)
+    with capi.BatchContext(sb, precision=capi.NEXD_FP32, compensated=True) as b:
+        b.reset()
+        b.run(1, sb.base.nt)
+        got = [b.seismograms(k) for k in range(sb.nshot)]
+    for k, s in enumerate(sb.shots):
+        err = rel_l2(got[k], fp64_single(s))
+        print(f"  rk3 shot {k}: FP32 compensated {err.max():.3e}", flush=True)
+        assert err.max() <= 1e-4
