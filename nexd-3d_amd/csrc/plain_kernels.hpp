@@ -335,6 +335,7 @@ stage_kernel(StageArgs<R> a) {
     constexpr int NP = S::NP, NPF = S::NPF, NFP = S::NFP, EB = S::EB, NT = S::NT;
     constexpr int NCOL = S::NCOL, NV = S::NV, NFC = S::NFC, NFCP = S::NFCP, NTHR = S::NTHR;
     __shared__ __align__(16) R lds[S::LDS];
+    __shared__ int sElem[EB];             // the block's element ids, for the face phase
     R *sQ = lds;                          // [9][NT]
     R *sX = lds + 9 * NT;                 // [24][NT]    PML flux columns
     R *sF = lds + (9 + (PML ? 24 : 0)) * NT;   // [EB][NFP][NFCP]
@@ -371,6 +372,16 @@ stage_kernel(StageArgs<R> a) {
     const bool valid = node_thr && idx < a.nlist;
     const int e = valid ? a.elist[idx] : 0;
     const size_t g = (size_t)e * NP + j;
+    if (node_thr && j == 0) sElem[eb] = e;
+    // first source of the element and its receiver slot, loaded with the
+    // first batch instead of at the assembly
+    int src_first = -1, rec_slot_e = -1;
+    if (node_thr) {
+        const int h = a.src_head[(size_t)shot * (a.nglob / NP) + e];
+        const int r = a.rec_slot[e];
+        src_first = valid ? h : -1;
+        rec_slot_e = valid ? r : -1;
+    }
 
     // One straight-line block per node thread, so the scheduler can issue
     // the loads ahead of the stores: q, metric and material, then (PML) the
@@ -459,7 +470,7 @@ stage_kernel(StageArgs<R> a) {
             for (int f = 0; f < NFCP; ++f) dst[f] = 0;
             continue;
         }
-        const int fe = a.elist[fidx];
+        const int fe = sElem[fb];
         const int fi = k / NPF, i = k - fi * NPF;
         const int own = fb * NP + a.fmask[fi * NPF + i];
         const int ft = a.ftype[fe * 4 + fi];
@@ -761,10 +772,9 @@ stage_kernel(StageArgs<R> a) {
         for (int f = 0; f < 9; ++f) rhs[f] = rhs[f] + lf[f];
         // sources (src/timeloopMod.f90:467-475)
         if (valid) {
-            const int *head = a.src_head + (size_t)shot * a.nglob / NP;
             const int *next = a.src_next + (size_t)shot * a.nsrc;
             const int *stype = a.srctype + (size_t)shot * a.nsrc;
-            for (int s = head[e]; s >= 0; s = next[s]) {
+            for (int s = src_first; s >= 0; s = next[s]) {
                 if (a.src_factored) {
                     // spatial operand times the source time function of the step
                     const R *sp = a.src_spatial + (((size_t)shot * a.nsrc + s) * 6) * NP + j;
@@ -823,7 +833,7 @@ stage_kernel(StageArgs<R> a) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) a.u_full[c * ng + g] = a.u_full[c * ng + g] + a.dt * qnew[6 + c];
             }
-            const int slot = a.rec_slot[e];
+            const int slot = rec_slot_e;
             if (slot >= 0) {
                 const size_t so = (size_t)shot * a.nslot + slot;
 #pragma unroll
