@@ -374,52 +374,52 @@ stage_kernel(StageArgs<R> a) {
     const size_t g = (size_t)e * NP + j;
     if (node_thr && j == 0) sElem[eb] = e;
     // first source of the element and its receiver slot, loaded with the
-    // first batch instead of at the assembly
+    // first batch instead of at the assembly (read only where valid)
     int src_first = -1, rec_slot_e = -1;
-    if (node_thr) {
-        const int h = a.src_head[(size_t)shot * (a.nglob / NP) + e];
-        const int r = a.rec_slot[e];
-        src_first = valid ? h : -1;
-        rec_slot_e = valid ? r : -1;
-    }
 
-    // One straight-line block per node thread, so the scheduler can issue
-    // the loads ahead of the stores: q, metric and material, then (PML) the
-    // coefficients and the three directions' auxiliaries, last-element fluxes
-    // and RK bases, each direction loaded before the previous one is
-    // computed.  No load is guarded by a branch: padding threads read element
+    // Phase 1 issues every load of the node threads in one batch: q, metric
+    // and material and (PML) the coefficients and all 27 auxiliaries and RK
+    // bases.  No load is guarded by a branch: padding threads read element
     // 0, and in stage 1 the RK base buffer is the input buffer (the value is
-    // not used).
+    // not used).  The last element's fluxes, the same for every element,
+    // are staged once per block in the face-value region of LDS, which is
+    // free until the face phase.
     R qs[9];
     R ef0 = 0, ef1 = 0, ef2 = 0, ef3 = 0, rho = 1;
+    R qv[9], mv[9], m2 = 1, m3 = 0, m4 = 0;
+    R pc[PML ? 9 : 1], cur[PML ? 27 : 1], bse[PML ? 27 : 1];
+    const R *pin = a.pin + (size_t)shot * 27 * ng;
+    const R *pbase = a.pbase + (size_t)shot * 27 * ng;
+    R *pout = a.pout + (size_t)shot * 27 * ng;
+    R *sLast = sF;                        // [27][NP] during phase 1
+    static_assert(!PML || 27 * NP <= EB * NFP * NFCP, "last-element fluxes must fit the face region");
     if (node_thr) {
-        R qv[9], mv[9];
+        src_first = a.src_head[(size_t)shot * (a.nglob / NP) + e];
+        rec_slot_e = a.rec_slot[e];
 #pragma unroll
         for (int f = 0; f < 9; ++f) qv[f] = qin[f * ng + g];
 #pragma unroll
         for (int k = 0; k < 9; ++k) mv[k] = a.metric[k * ng + g];
         const R *mc = matc + (size_t)e * 8;
-        const R m2 = mc[2], m3 = mc[3], m4 = mc[4];
-        // PML inputs (unused and not loaded without PML)
-        const R *pmlc = a.pmlc + (size_t)shot * a.pmlc_stride_ng * ng;
-        const R *pin = a.pin + (size_t)shot * 27 * ng;
-        const R *pbase = a.pbase + (size_t)shot * 27 * ng;
-        R *pout = a.pout + (size_t)shot * 27 * ng;
-        const R *lastf = a.last_flux + (size_t)shot * 27 * NP;
-        R pc[9], cur[3][9], use[3][9], bse[3][9];
-        auto load_dir = [&](int d) {
-#pragma unroll
-            for (int i = 0; i < 9; ++i) {
-                cur[d][i] = pin[(d * 9 + i) * ng + g];
-                use[d][i] = lastf[(d * 9 + i) * NP + j];
-                bse[d][i] = pbase[(d * 9 + i) * ng + g];
-            }
-        };
+        m2 = mc[2]; m3 = mc[3]; m4 = mc[4];
         if constexpr (PML) {
+            const R *pmlc = a.pmlc + (size_t)shot * a.pmlc_stride_ng * ng;
 #pragma unroll
             for (int k = 0; k < 9; ++k) pc[k] = pmlc[k * ng + g];
-            load_dir(0);
+#pragma unroll
+            for (int c = 0; c < 27; ++c) cur[c] = pin[c * ng + g];
+#pragma unroll
+            for (int c = 0; c < 27; ++c) bse[c] = pbase[c * ng + g];
         }
+    }
+    if constexpr (PML) {
+        if (a.pml_compat) {
+            const R *lastf = a.last_flux + (size_t)shot * 27 * NP;
+            for (int i = t; i < 27 * NP; i += NTHR) sLast[i] = lastf[i];
+        }
+        __syncthreads();
+    }
+    if (node_thr) {
 #pragma unroll
         for (int f = 0; f < 9; ++f) {
             qs[f] = valid ? qv[f] : R(0);
@@ -438,23 +438,29 @@ stage_kernel(StageArgs<R> a) {
             R Fm[9], Gm[9], Hm[9];
             elastic_flux_node(qs, ef0, ef1, ef2, ef3, Fm, Gm, Hm);
             int col = 0;
+            R pv[27];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                if (d + 1 < 3) load_dir(d + 1);
                 const R al = valid ? pc[d] : R(0), dk = valid ? pc[3 + d] : R(0), kk = valid ? pc[6 + d] : R(1);
                 R *FF = d == 0 ? Fm : (d == 1 ? Gm : Hm);
                 const int skip = d == 0 ? 4 : (d == 1 ? 5 : 3);
 #pragma unroll
                 for (int i = 0; i < 9; ++i) {
-                    const R c0 = valid ? cur[d][i] : R(0);
+                    const int c = d * 9 + i;
+                    const R c0 = valid ? cur[c] : R(0);
                     const R mod = (c0 + FF[i]) / kk;
                     if (i != skip) sX[(col++) * NT + t] = mod;
-                    const R u = a.pml_compat ? use[d][i] : mod;
+                    const R u = a.pml_compat ? sLast[c * NP + j] : mod;
                     const R dadt = -al * c0 - dk * (c0 + u);
-                    const R b0 = valid ? bse[d][i] : R(0);
-                    const R v = rk_combine<R>(a.scheme, a.irk, c0, b0, dadt, a.dt, a.w13, a.w23);
-                    if (valid) pout[(d * 9 + i) * ng + g] = v;
+                    const R b0 = valid ? bse[c] : R(0);
+                    pv[c] = rk_combine<R>(a.scheme, a.irk, c0, b0, dadt, a.dt, a.w13, a.w23);
                 }
+            }
+            // one guarded block of stores (a guard per store leaves waits
+            // for every earlier store at the joins)
+            if (valid) {
+#pragma unroll
+                for (int c = 0; c < 27; ++c) pout[c * ng + g] = pv[c];
             }
         }
     }
