@@ -39,7 +39,9 @@ struct Batch : BatchBase {
     int np = 0, npf = 0, nfp = 0, nrk = 0, nslot = 0;
     size_t ng = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;               // interior list during graph capture
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<void *> allocs;
     R *q[3] = {nullptr, nullptr, nullptr};
     R *qlo[3] = {nullptr, nullptr, nullptr};   // low parts (compensated state)
@@ -73,6 +75,9 @@ struct Batch : BatchBase {
         for (void *p : allocs) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -124,8 +129,11 @@ struct Batch : BatchBase {
         ng = (size_t)c.nelem * np;
         nrk = c.timeint == 1 ? 1 : (c.timeint == 2 ? 2 : 3);
         if (hipSetDevice(c.device) != hipSuccess) return fail(NEXD_E_HIP, "hipSetDevice failed");
-        if (hipStreamCreate(&stream) != hipSuccess) return fail(NEXD_E_HIP, "hipStreamCreate failed");
-        if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess)
+        if (hipStreamCreate(&stream) != hipSuccess || hipStreamCreate(&side) != hipSuccess)
+            return fail(NEXD_E_HIP, "hipStreamCreate failed");
+        if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess ||
+            hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&ev_join, hipEventDisableTiming) != hipSuccess)
             return fail(NEXD_E_HIP, "hipEventCreate failed");
         const size_t S = (size_t)c.nshot;
         int rc = 0;
@@ -208,29 +216,30 @@ struct Batch : BatchBase {
     }
 
     template <int ORDER, bool ATT, bool PML>
-    void launch_one(const nexd::StageArgs<R> &a) {
+    void launch_one(const nexd::StageArgs<R> &a, hipStream_t s) {
         using Sh = nexd::Shape<ORDER, ATT, PML, R>;
         const unsigned grid = (unsigned)((a.nlist + Sh::EB - 1) / Sh::EB) * (unsigned)a.nshot;
         if (grid == 0) return;
-        hipLaunchKernelGGL((nexd::stage_kernel<ORDER, ATT, PML, R>), dim3(grid), dim3(Sh::NTHR), 0, stream, a);
+        hipLaunchKernelGGL((nexd::stage_kernel<ORDER, ATT, PML, R>), dim3(grid), dim3(Sh::NTHR), 0, s, a);
     }
 
     template <int ORDER>
-    void stage(const nexd::StageArgs<R> &a, bool pml) {
+    void stage(const nexd::StageArgs<R> &a, bool pml, hipStream_t s) {
         const bool att = cfg.attenuation != 0;
-        if (att && pml) launch_one<ORDER, true, true>(a);
-        else if (att) launch_one<ORDER, true, false>(a);
-        else if (pml) launch_one<ORDER, false, true>(a);
-        else launch_one<ORDER, false, false>(a);
+        if (att && pml) launch_one<ORDER, true, true>(a, s);
+        else if (att) launch_one<ORDER, true, false>(a, s);
+        else if (pml) launch_one<ORDER, false, true>(a, s);
+        else launch_one<ORDER, false, false>(a, s);
     }
 
-    void dispatch(const nexd::StageArgs<R> &a, bool pml) {
+    void dispatch(const nexd::StageArgs<R> &a, bool pml) { dispatch(a, pml, stream); }
+    void dispatch(const nexd::StageArgs<R> &a, bool pml, hipStream_t s) {
         switch (cfg.order) {
-            case 1: stage<1>(a, pml); break;
-            case 2: stage<2>(a, pml); break;
-            case 3: stage<3>(a, pml); break;
-            case 4: stage<4>(a, pml); break;
-            default: stage<5>(a, pml); break;
+            case 1: stage<1>(a, pml, s); break;
+            case 2: stage<2>(a, pml, s); break;
+            case 3: stage<3>(a, pml, s); break;
+            case 4: stage<4>(a, pml, s); break;
+            default: stage<5>(a, pml, s); break;
         }
     }
 
@@ -299,15 +308,24 @@ struct Batch : BatchBase {
         const bool pml = cfg.pml && n_pml > 0;
         for (int irk = 1; irk <= nrk; ++irk) {
             set_stage_buffers(a, irk);
-            if (pml && a.pml_compat) last_flux_dispatch(a.qin, a.pin);
+            // the interior list runs on a side stream, beside the PML list:
+            // both read the stage input and write disjoint elements
+            const bool fork = pml && n_plain > 0;
+            if (fork) {
+                (void)hipEventRecord(ev_fork, stream);
+                (void)hipStreamWaitEvent(side, ev_fork, 0);
+            }
             a.elist = elist_plain;
             a.nlist = n_plain;
-            dispatch(a, false);
+            dispatch(a, false, fork ? side : stream);
+            if (fork) (void)hipEventRecord(ev_join, side);
             if (pml) {
+                if (a.pml_compat) last_flux_dispatch(a.qin, a.pin);
                 a.elist = elist_pml;
                 a.nlist = n_pml;
                 dispatch(a, true);
             }
+            if (fork) (void)hipStreamWaitEvent(stream, ev_join, 0);
             if (cfg.timeint == NEXD_EULER) {
                 (void)hipMemcpyAsync(q[0], q[1], sizeof(R) * 9 * ng * cfg.nshot, hipMemcpyDeviceToDevice, stream);
                 if (cfg.compensated)

@@ -71,6 +71,8 @@ struct nexd_ctx {
          have_src = false, have_rec = false, initialized = false;
     hipGraphExec_t graph = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipStream_t side = nullptr;                 // interior list during graph capture
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     float last_ms = 0.f;
 };
 
@@ -263,19 +265,28 @@ void enqueue_step(nexd_ctx *c) {
     for (int irk = 1; irk <= nrk; ++irk) {
         stage_buffers(c, irk, a);
         const double *qin = a.qin, *pin = a.pin;
-        if (pml && a.pml_compat) dispatch_last_flux(c, qin, pin);
         const bool energy = a.last_stage && c->cfg.energy && pml;
-        // elements without PML (every element when PML is off)
+        // elements without PML (every element when PML is off); with PML they
+        // run on a side stream, beside the PML list: both read the stage
+        // input and write disjoint elements and energy partials
+        const bool fork = pml && c->n_plain > 0;
+        if (fork) {
+            (void)hipEventRecord(c->ev_fork, c->stream);
+            (void)hipStreamWaitEvent(c->side, c->ev_fork, 0);
+        }
         a.elist = c->elist_plain;
         a.nlist = c->n_plain;
         a.blk_energy = energy ? c->blk_energy : nullptr;
-        dispatch_stage(c->cfg.order, a, att, false, c->stream);
+        dispatch_stage(c->cfg.order, a, att, false, fork ? c->side : c->stream);
+        if (fork) (void)hipEventRecord(c->ev_join, c->side);
         if (pml) {
+            if (a.pml_compat) dispatch_last_flux(c, qin, pin);
             a.elist = c->elist_pml;
             a.nlist = c->n_pml;
             a.blk_energy = energy ? c->blk_energy + nblocks(c->cfg.order, c->n_plain, att, false) : nullptr;
             dispatch_stage(c->cfg.order, a, att, true, c->stream);
         }
+        if (fork) (void)hipStreamWaitEvent(c->stream, c->ev_join, 0);
         if (scheme == NEXD_EULER)
             (void)hipMemcpyAsync(c->q[0], c->q[1], sizeof(double) * 9 * c->nglob, hipMemcpyDeviceToDevice, c->stream);
     }
@@ -335,8 +346,11 @@ int nexd_create(const nexd_config *cfg, nexd_ctx **out) {
     c->nrk = cfg->timeint == 1 ? 1 : (cfg->timeint == 2 ? 2 : 3);
     auto bail = [&](int rc) { g_create_error = c->err; nexd_destroy(c); return rc; };
     if (hipSetDevice(cfg->device) != hipSuccess) return bail(fail(c, NEXD_E_HIP, "hipSetDevice failed"));
-    if (hipStreamCreate(&c->stream) != hipSuccess) return bail(fail(c, NEXD_E_HIP, "hipStreamCreate failed"));
-    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+    if (hipStreamCreate(&c->stream) != hipSuccess || hipStreamCreate(&c->side) != hipSuccess)
+        return bail(fail(c, NEXD_E_HIP, "hipStreamCreate failed"));
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
         return bail(fail(c, NEXD_E_HIP, "hipEventCreate failed"));
     const size_t ng = c->nglob;
     int rc = 0;
@@ -711,6 +725,9 @@ void nexd_destroy(nexd_ctx *c) {
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
