@@ -1,4 +1,5 @@
-// Stage kernels of the single-shot time loop ("plain" path, FP64 or FP32).
+// Stage kernels of the time loop, shared by the single-shot FP64 path and the
+// batched FP32/FP64 path.
 //
 // One kernel launch advances one Runge-Kutta stage for a list of elements:
 // it fuses what the reference does in its element loop and its separate RK
@@ -21,7 +22,9 @@
 // Layout in HBM: fields are structure-of-arrays, element-major within a field
 // (the reference's q(nglob,9) column-major), so the Np nodes of an element are
 // contiguous in every field and a block of consecutive elements is one
-// contiguous run per field.  Threads map to (element-in-block, node).
+// contiguous run per field.  Node phases map threads to (element-in-block,
+// node); the volume and LIFT products run on the matrix cores (16x16x4 MFMA)
+// over all four waves of the block.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -32,8 +35,6 @@ struct Dims {
     static constexpr int NP = (ORDER + 1) * (ORDER + 2) * (ORDER + 3) / 6;
     static constexpr int NPF = (ORDER + 1) * (ORDER + 2) / 2;
     static constexpr int NFP = 4 * NPF;
-    static constexpr int EB = (256 / NP) > 0 ? (256 / NP) : 1;   // elements per block
-    static constexpr int NT = EB * NP;                          // threads per block
 };
 
 constexpr int NMB = 3;

@@ -1,7 +1,7 @@
 """Named test workloads shared by the golden-fixture script and the tests.
 
 Each entry maps a name to (builder, kwargs) of nexd_amd.problem; the
-builders are deterministic, so the GPU box regenerates the same inputs.
+builders are deterministic, so a GPU run regenerates the same inputs.
 """
 from nexd_amd import problem
 
