@@ -58,10 +58,15 @@ def locate(points, mesh, geom, order: int) -> Located:
         dist = np.sqrt((gx - px) ** 2 + (gy - py) ** 2 + (gz - pz) ** 2)   # (nelem, Np)
         excluded = np.zeros(nelem, dtype=bool)
         found = False
-        for _attempt in range(20):
+        # like the reference, keep excluding elements until one holds the
+        # point (src/sourceReceiverMod.f90:242-330); many elements tie for the
+        # nearest node when it is a mesh vertex
+        for _attempt in range(nelem):
             d = np.where(excluded[:, None], np.inf, dist)
             flat = int(np.argmin(d))               # first occurrence of the minimum
             ie, i = divmod(flat, d.shape[1])
+            if excluded[ie]:
+                break
             sr, ss, st = r0[i], s0[i], t0[i]
             c = mesh.coord[mesh.elem[ie]]
             rx, ry, rz, sx, sy, sz, tx, ty, tz, _ = _metric(ref, gx[ie:ie + 1], gy[ie:ie + 1], gz[ie:ie + 1])
