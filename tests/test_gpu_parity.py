@@ -141,3 +141,23 @@ def test_restart_from_state():
         got = second.seismograms()
     assert np.array_equal(got[:, :6, 30:], ref[:, :6, 30:])
     assert np.array_equal(got[:, 6:, 31:], ref[:, 6:, 31:])
+
+
+@pytest.mark.parametrize("order,timeint,nt", [(1, 2, 25), (2, 3, 60), (5, 2, 60)])
+def test_orders_with_attenuation_and_pml(order, timeint, nt):
+    """Every order's block shape and MFMA tiling (N=1: 4 nodes, one k-step;
+    N=5: 56 nodes, 14 k-steps, 4 LIFT tiles) with attenuation and PML, FP64
+    against the oracle.  The reference's CFL step is unstable at N=1, so that
+    run is short."""
+    from nexd_amd import problem
+    p = problem.layered_box(nx=3, ny=3, nz=3, order=order, nt=nt, pml_cells=1, nrec_side=2, timeint=timeint,
+                            attenuation=True)
+    arrs = layout.reference_arrays(p)
+    print(f"[gpu orders] N={order} timeint={timeint} nelem={p.nelem} nt={nt}", flush=True)
+    with capi.Context(arrs, compat=capi.COMPAT_ALL) as ctx:
+        ctx.reset()
+        ctx.run(1, nt)
+        got = ctx.seismograms()
+    ref, _, _ = oracle.run(arrs, nt)
+    err = np.linalg.norm(got - ref, axis=-1) / np.maximum(np.linalg.norm(ref, axis=-1), 1e-300)
+    assert err.max() <= 1e-10, err.max()
