@@ -12,6 +12,12 @@ step (memory variables and PML auxiliaries are work, not DOF).
 
 Prints one JSON line (see the fields below).  --gpus N > 1 is refused: this
 build runs one GPU.
+
+Other workloads, each printing its own JSON line instead:
+  --multishot K  K shots on the 3,072-tet N=3 cube in one FP32 batch against
+                 the FP64 plain path, in shot-steps/s;
+  --uneven K     K independent instances of different size and order as one
+                 group against the same instances one by one.
 """
 from __future__ import annotations
 
@@ -159,6 +165,52 @@ def accuracy_run(args, pml_cells):
     return out
 
 
+def multishot_main(args):
+    """K shots on the 3,072-tet N=3 homogeneous cube (8^3 hexes, 10 km), per
+    shot a +-5% Gaussian-correlated velocity perturbation (1 km) and a random
+    moment tensor (M0 log-uniform 1e13..1e16 N m), 64 surface receivers,
+    RK2, nt = warmup + steps.  All K shots run in one FP32 batch; the FP64
+    plain path runs shot 0 alone.  Rates are shot-steps per second (a
+    shot-seismogram is nt shot-steps)."""
+    import numpy as np
+    from nexd_amd import batch, capi, layout
+    nt = args.warmup + args.steps
+    t0 = time.time()
+    sb = batch.multishot_cube(args.multishot, n=8, order=3, nt=nt, timeint=2, full_source_arrays=True)
+    p = sb.shots[0]
+    log(f"[bench] multishot set-up {time.time() - t0:.1f}s: {sb.nshot} shots, nelem={p.nelem} N={p.order}")
+    with capi.Context(layout.reference_arrays(p), compat=capi.COMPAT_ALL) as ctx:
+        ctx.reset()
+        if args.warmup:
+            ctx.run(1, args.warmup)
+        ctx.run(args.warmup + 1, nt)
+        ms64 = ctx.last_run_ms() / args.steps
+        ref = ctx.seismograms()
+    with capi.BatchContext(sb, precision=capi.NEXD_FP32, compat=capi.COMPAT_ALL) as b:
+        b.reset()
+        if args.warmup:
+            b.run(1, args.warmup)
+        b.run(args.warmup + 1, nt)
+        ms32 = b.last_run_ms() / args.steps
+        got = b.seismograms(0)
+    err = rel_l2_max(got, ref)
+    rate32 = sb.nshot / (ms32 * 1e-3)
+    rate64 = 1.0 / (ms64 * 1e-3)
+    dof = p.nelem * p.ref.np_ * 9
+    log(f"[bench] multishot: FP32 batch {ms32:.3f} ms/step for {sb.nshot} shots, FP64 plain {ms64:.3f} ms/step "
+        f"for one; shot-steps/s {rate32:.4g} vs {rate64:.4g}; FP32 error vs FP64 (shot 0) {err:.2e}")
+    print(json.dumps({
+        "metric": "shot-seismograms/s on 1 GPU: FP32 batched vs FP64 plain (shot-steps/s; one seismogram = nt)",
+        "value": rate32, "unit": "shot-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms32, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{sb.nshot} shots on the 3,072-tet N=3 cube (8^3 hexes x 6), +-5% correlated "
+                               f"velocity perturbation and random moment tensor per shot, 64 receivers, RK2",
+                   "dof_per_step": dof * sb.nshot, "dof_updates_per_s": dof * rate32,
+                   "fp64_plain_shot_steps_per_s": rate64, "fp32_over_fp64_shot_throughput": rate32 / rate64,
+                   "fp32_seismogram_rel_l2_vs_fp64": err}}), flush=True)
+
+
 def uneven_main(args):
     """Uneven batch: K independent instances of different size and order
     (layered elastic, one point force each), FP32, nt = warmup + steps.  The
@@ -217,6 +269,9 @@ def main():
                          "and compensated) and report the seismogram error of each FP32 form against FP64; 0 skips")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (calibrates HBM byte counters)")
+    ap.add_argument("--multishot", type=int, default=0, metavar="K",
+                    help="instead: K shots on the 3,072-tet N=3 cube (per-shot +-5%% velocity perturbation, random "
+                         "moment tensors), FP32 batched vs the FP64 plain path, in shot-seismograms/s")
     ap.add_argument("--uneven", type=int, default=0, metavar="K",
                     help="instead: K uneven instances (6n^3 tets, n in 10..30, N in 2..5, FP32) advanced as one "
                          "group vs one by one")
@@ -233,6 +288,9 @@ def main():
         sys.exit(2)
     if args.uneven:
         uneven_main(args)
+        return
+    if args.multishot:
+        multishot_main(args)
         return
     if args.copy_probe:
         import torch
