@@ -164,13 +164,38 @@ __device__ __forceinline__ void upwind_flux(const R *fr, R vp, R vs, R rho, R la
     }
 }
 
+// One Runge-Kutta stage update (src/rungekuttaMod.f90:90-120):
+//   stage 1 (and Euler):   a + dt*rhs
+//   TVD RK2, stage 2:      ((base + a) + dt*rhs) / 2
+//   TVD RK3, stage 2:      3/4 base + 1/4 (a + dt*rhs)
+//   TVD RK3, stage 3:      w13 base + w23 (a + dt*rhs)
+// The stage is uniform over a launch, so its weights are chosen once
+// (rk_stage) and every combined value is the same straight-line code: both
+// forms are computed and one is selected.  A branch per value would split a
+// node's independent updates into separate basic blocks, which the compiler
+// then cannot interleave, and each update's latency would add up.  dt*rhs
+// and a + dt*rhs are rounded separately; the weighted sum is one FMA (exact
+// for stage 1: 0 * base + x).
 template <typename R>
-__device__ __forceinline__ R rk_combine(int scheme, int irk, R a, R base, R rhs, R dt, R w13, R w23) {
-    // src/rungekuttaMod.f90:90-120
-    if (irk == 1) return a + rhs * dt;
-    if (scheme == 2) return R(0.5) * (base + a + rhs * dt);
-    if (irk == 2) return R(0.75) * base + R(0.25) * (a + rhs * dt);
-    return w13 * base + w23 * (a + rhs * dt);
+struct RkStage {
+    R wb, wx;    // wb * base + wx * (a + dt*rhs)
+    bool avg;    // TVD RK2 stage 2
+};
+template <typename R>
+__device__ __forceinline__ RkStage<R> rk_stage(int scheme, int irk, R w13, R w23) {
+    if (irk == 1) return {R(0), R(1), false};
+    if (scheme == 2) return {R(0.5), R(0.5), true};
+    if (irk == 2) return {R(0.75), R(0.25), false};
+    return {w13, w23, false};
+}
+template <typename R>
+__device__ __forceinline__ R rk_combine(const RkStage<R> &s, R a, R base, R rhs, R dt) {
+#pragma clang fp contract(off)
+    const R k = rhs * dt;
+    const R x = a + k;
+    const R y = ((base + a) + k) * R(0.5);
+    const R z = __builtin_fma(s.wb, base, s.wx * x);
+    return s.avg ? y : z;
 }
 
 // ---------------------------------------------------------------------------
@@ -253,7 +278,11 @@ struct Shape {
     static constexpr int NCOL = PML ? (ATT ? 27 : 24) : 9;     // columns per element
     static constexpr int NV = PML ? (ATT ? 33 : 24) : 27;      // volume values per node
     static constexpr int NFC = ATT ? 15 : 9;                    // face values per face node
-    static constexpr int NFCP = ATT ? 16 : 10;                  // padded for 16-byte LDS reads
+    static constexpr int NFCP = ATT ? 16 : 10;                  // face-value rows in LDS
+    // LDS row lengths, odd so that threads reading or writing one value of
+    // consecutive rows (one row per lane) fall in different banks
+    static constexpr int SFN = NFP | 1;                         // face values: row of face nodes
+    static constexpr int sln(int eb) { return (eb * NP) | 1; }  // lifted values: row of block nodes
     // four waves per block: the node phases use EB*NP threads of them, the
     // volume term all four
     static constexpr int NTHR = 256;
@@ -262,11 +291,11 @@ struct Shape {
     static constexpr int BPC = sizeof(R) == 8 ? 2 : 4;
     static constexpr int LDS_BYTES = 160 * 1024 / BPC;
     // LDS regions, in reals: q and PML flux columns (a, from 0) and face
-    // values (b, after a) until the operand fragments are in registers; then
-    // volume values (c, from 0) and the lifted face values (16 per node, after
-    // max(a, c)); the metric (m) lives from the first phase to the assembly,
-    // after all of them; the energy values and the block reduction alias
-    // everything at the end
+    // values (b, after a; [EB][NFCP][SFN]) until the operand fragments are in
+    // registers; then volume values (c, from 0) and the lifted face values
+    // ([16][sln], after max(a, c)); the metric (m) lives from the first phase
+    // to the assembly, after all of them; the energy values and the block
+    // reduction alias everything at the end
     static constexpr int off_lift(int eb) {
         const int nt = eb * NP;
         const int a = (9 + (PML ? 24 : 0)) * nt;
@@ -275,8 +304,8 @@ struct Shape {
     }
     static constexpr int off_metric(int eb) {
         const int nt = eb * NP;
-        const int ab = (9 + (PML ? 24 : 0)) * nt + eb * NFP * NFCP;
-        const int l = off_lift(eb) + 16 * nt;
+        const int ab = (9 + (PML ? 24 : 0)) * nt + eb * NFCP * SFN;
+        const int l = off_lift(eb) + 16 * sln(eb);
         return ab > l ? ab : l;
     }
     static constexpr int lds_reals(int eb) {
@@ -335,14 +364,15 @@ stage_kernel(StageArgs<R> a) {
     using S = Shape<ORDER, ATT, PML, R>;
     constexpr int NP = S::NP, NPF = S::NPF, NFP = S::NFP, EB = S::EB, NT = S::NT;
     constexpr int NCOL = S::NCOL, NV = S::NV, NFC = S::NFC, NFCP = S::NFCP, NTHR = S::NTHR;
+    constexpr int SFN = S::SFN, SLN = S::sln(S::EB);
     __shared__ __align__(16) R lds[S::LDS];
     __shared__ int sElem[EB];             // the block's element ids, for the face phase
     R *sQ = lds;                          // [9][NT]
     R *sX = lds + 9 * NT;                 // [24][NT]    PML flux columns
-    R *sF = lds + (9 + (PML ? 24 : 0)) * NT;   // [EB][NFP][NFCP]
+    R *sF = lds + (9 + (PML ? 24 : 0)) * NT;   // [EB][NFCP][SFN]  face values, face-node rows
     R *sV = lds;                          // [NT][NV]    after the columns are in registers
     R *sM = lds + S::OFF_M;               // [9][NT]     metric of the block's nodes
-    R *sL = lds + S::OFF_L;               // [NT][16]    lifted face values
+    R *sL = lds + S::OFF_L;               // [16][SLN]   lifted face values, node rows
     constexpr int NRED = NTHR <= 64 ? 64 : (NTHR <= 128 ? 128 : 256);   // power of two >= NTHR
     static_assert(NRED == NTHR, "the block reduction assumes a power-of-two block");
     R *sRed = lds + 15 * NT;              // [NTHR]      energy reduction
@@ -393,7 +423,7 @@ stage_kernel(StageArgs<R> a) {
     const R *pbase = a.pbase + (size_t)shot * 27 * ng;
     R *pout = a.pout + (size_t)shot * 27 * ng;
     R *sLast = sF;                        // [27][NP] during phase 1
-    static_assert(!PML || 27 * NP <= EB * NFP * NFCP, "last-element fluxes must fit the face region");
+    static_assert(!PML || 27 * NP <= EB * NFCP * S::SFN, "last-element fluxes must fit the face region");
     if (node_thr) {
         src_first = a.src_head[(size_t)shot * (a.nglob / NP) + e];
         rec_slot_e = a.rec_slot[e];
@@ -420,48 +450,58 @@ stage_kernel(StageArgs<R> a) {
         }
         __syncthreads();
     }
+    // Node threads past the end of the list (last block only) carry element
+    // 0's values through the node phases: finite, confined to their own
+    // columns, never stored.
     if (node_thr) {
 #pragma unroll
         for (int f = 0; f < 9; ++f) {
-            qs[f] = valid ? qv[f] : R(0);
+            qs[f] = qv[f];
             sQ[f * NT + t] = qs[f];
         }
 #pragma unroll
-        for (int k = 0; k < 9; ++k) sM[k * NT + t] = valid ? mv[k] : R(0);
-        if (valid) {
-            rho = m2;
-            ef0 = -(m3 + 2 * m4); ef1 = -m3; ef2 = -m4; ef3 = R(-1.0) / rho;
-        }
+        for (int k = 0; k < 9; ++k) sM[k * NT + t] = mv[k];
+        rho = m2;
+        ef0 = -(m3 + 2 * m4); ef1 = -m3; ef2 = -m4; ef3 = R(-1.0) / rho;
         if constexpr (PML) {
             // modified fluxes (src/timeloopMod.f90:399-405) and the ADE update
             // of f',g',h' (src/rungekuttaMod.f90:52-63); the update needs only
             // stage-start values, so it is done here and the auxiliaries die
             R Fm[9], Gm[9], Hm[9];
             elastic_flux_node(qs, ef0, ef1, ef2, ef3, Fm, Gm, Hm);
+            const RkStage<R> rs = rk_stage<R>(a.scheme, a.irk, a.w13, a.w23);
             int col = 0;
-            R pv[27];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                const R al = valid ? pc[d] : R(0), dk = valid ? pc[3 + d] : R(0), kk = valid ? pc[6 + d] : R(1);
+                R pv[9];
+                const R al = pc[d], dk = pc[3 + d];
+                // (f' + F) / k as a product with 1/k: one division per
+                // direction instead of nine, within an ulp of the quotient
+                const R rk = R(1) / pc[6 + d];
                 R *FF = d == 0 ? Fm : (d == 1 ? Gm : Hm);
                 const int skip = d == 0 ? 4 : (d == 1 ? 5 : 3);
 #pragma unroll
                 for (int i = 0; i < 9; ++i) {
                     const int c = d * 9 + i;
-                    const R c0 = valid ? cur[c] : R(0);
-                    const R mod = (c0 + FF[i]) / kk;
+                    const R c0 = cur[c];
+                    const R mod = (c0 + FF[i]) * rk;
                     if (i != skip) sX[(col++) * NT + t] = mod;
-                    const R u = a.pml_compat ? sLast[c * NP + j] : mod;
+                    // read unconditionally and selected (the region is
+                    // stale without compat): no branch between components
+                    const R lastv = sLast[c * NP + j];
+                    const R u = a.pml_compat ? lastv : mod;
                     const R dadt = -al * c0 - dk * (c0 + u);
-                    const R b0 = valid ? bse[c] : R(0);
-                    pv[c] = rk_combine<R>(a.scheme, a.irk, c0, b0, dadt, a.dt, a.w13, a.w23);
+                    pv[i] = rk_combine<R>(rs, c0, bse[c], dadt, a.dt);
                 }
-            }
-            // one guarded block of stores (a guard per store leaves waits
-            // for every earlier store at the joins)
-            if (valid) {
+                // one guarded block of stores per direction (a guard per
+                // store leaves waits for every earlier store at the joins)
+                if (valid) {
 #pragma unroll
-                for (int c = 0; c < 27; ++c) pout[c * ng + g] = pv[c];
+                    for (int i = 0; i < 9; ++i) pout[(d * 9 + i) * ng + g] = pv[i];
+                }
+                // the nine updates of a direction interleave; a scheduling
+                // fence per direction keeps the working set in registers
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
     }
@@ -471,10 +511,12 @@ stage_kernel(StageArgs<R> a) {
     for (int it = t; it < EB * NFP; it += NTHR) {
         const int fb = it / NFP, k = it - fb * NFP;
         const int fidx = eblk * EB + fb;
-        R *dst = sF + (fb * NFP + k) * NFCP;
+        // value f of this face node at dst[f * SFN]: consecutive threads
+        // write consecutive words
+        R *dst = sF + fb * NFCP * SFN + k;
         if (fidx >= a.nlist) {
 #pragma unroll
-            for (int f = 0; f < NFCP; ++f) dst[f] = 0;
+            for (int f = 0; f < NFC; ++f) dst[f * SFN] = 0;
             continue;
         }
         const int fe = sElem[fb];
@@ -497,13 +539,11 @@ stage_kernel(StageArgs<R> a) {
         const R sgn = ft == -2 ? R(-1) : R(1);
         const R fs = a.fscale[(size_t)fe * NFP + k];
 #pragma unroll
-        for (int f = 0; f < 9; ++f) dst[f] = fs * (sgn * fl[f]) / 2;
+        for (int f = 0; f < 9; ++f) dst[f * SFN] = fs * (sgn * fl[f]) / 2;
         if constexpr (ATT) {
 #pragma unroll
-            for (int f = 0; f < 6; ++f) dst[9 + f] = fs * (sgn * af[f]) / 2;
+            for (int f = 0; f < 6; ++f) dst[(9 + f) * SFN] = fs * (sgn * af[f]) / 2;
         }
-#pragma unroll
-        for (int f = NFC; f < NFCP; ++f) dst[f] = 0;
     }
     __syncthreads();
 
@@ -569,10 +609,10 @@ stage_kernel(StageArgs<R> a) {
         const int el = w + c * NW;
         const int f = lane & 15;
         const bool ok = el < EB && f < NFC;
-        const R *fp = sF + (ok ? el : 0) * NFP * NFCP + (ok ? f : 0);
+        const R *fp = sF + ((ok ? el : 0) * NFCP + (ok ? f : 0)) * S::SFN + (lane >> 4);
 #pragma unroll
         for (int ks = 0; ks < KSL; ++ks) {
-            const R v = fp[(ks * 4 + (lane >> 4)) * NFCP];
+            const R v = fp[ks * 4];
             fb[c][ks] = ok ? v : R(0);
         }
     }
@@ -610,7 +650,7 @@ stage_kernel(StageArgs<R> a) {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const int node = mt * 16 + (sizeof(R) == 8 ? 4 * v + (lane >> 4) : 4 * (lane >> 4) + v);
-                        if (node < NP) sL[(el * NP + node) * 16 + (lane & 15)] = acc[v];
+                        if (node < NP) sL[(lane & 15) * SLN + el * NP + node] = acc[v];
                     }
                 }
             }
@@ -708,7 +748,7 @@ stage_kernel(StageArgs<R> a) {
     R lf[NFC];
     if (node_thr) {
 #pragma unroll
-        for (int f = 0; f < NFC; ++f) lf[f] = sL[t * 16 + f];
+        for (int f = 0; f < NFC; ++f) lf[f] = sL[f * SLN + t];
     }
     R rhs[9];
     R gv[9];   // velocity gradients: gv[c*3+k] = d v_c / d x_k
@@ -750,25 +790,31 @@ stage_kernel(StageArgs<R> a) {
             const R ex = gv[0], ey = gv[4], ez = gv[8];
             const R exy = gv[3] + gv[1], eyz = gv[7] + gv[5], exz = gv[6] + gv[2];
             R *thout = a.thout + (size_t)shot * 18 * ng;
+            const RkStage<R> rs = rk_stage<R>(a.scheme, a.irk, a.w13, a.w23);
 #pragma unroll
             for (int i = 0; i < NMB; ++i) {
                 const R w = a.wl[i];
                 const R vol[6] = {-w * ex, -w * ey, -w * ez, R(-0.5) * w * exy, R(-0.5) * w * eyz, R(-0.5) * w * exz};
-#pragma unroll
-                for (int c = 0; c < 6; ++c) {
-                    const int k = i * 6 + c;
-                    const R arq = vol[c] + w * lf[9 + c] - w * th[k];
-                    const R nv = rk_combine<R>(a.scheme, a.irk, th[k], thb[k], arq, a.dt, a.w13, a.w23);
-                    if (valid) thout[k * ng + g] = nv;
-                }
+                R *tm = th + i * 6;
                 const R a1 = avc[i * 3 + 0], a2 = avc[i * 3 + 1], a3 = avc[i * 3 + 2];
-                const R *tm = th + i * 6;
                 rhs[0] = rhs[0] + a3 * tm[0] + a1 * (tm[1] + tm[2]);
                 rhs[1] = rhs[1] + a3 * tm[1] + a1 * (tm[2] + tm[0]);
                 rhs[2] = rhs[2] + a3 * tm[2] + a1 * (tm[0] + tm[1]);
                 rhs[3] = rhs[3] + a2 * tm[3];
                 rhs[4] = rhs[4] + a2 * tm[4];
                 rhs[5] = rhs[5] + a2 * tm[5];
+                // the stage-start values are used up: th takes the new ones
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    const int k = i * 6 + c;
+                    const R arq = vol[c] + w * lf[9 + c] - w * th[k];
+                    th[k] = rk_combine<R>(rs, th[k], thb[k], arq, a.dt);
+                }
+            }
+            // one guarded block of stores
+            if (valid) {
+#pragma unroll
+                for (int k = 0; k < 18; ++k) thout[k * ng + g] = th[k];
             }
         }
     }
@@ -824,11 +870,12 @@ stage_kernel(StageArgs<R> a) {
                 if (valid) { qout[f * ng + g] = qnew[f]; a.qlo_out[so + f * ng + g] = lo; }
             }
         } else {
+            const RkStage<R> rs = rk_stage<R>(a.scheme, a.irk, a.w13, a.w23);
 #pragma unroll
-            for (int f = 0; f < 9; ++f) {
-                const R base = qb[f];
-                qnew[f] = rk_combine<R>(a.scheme, a.irk, qs[f], base, rhs[f], a.dt, a.w13, a.w23);
-                if (valid) qout[f * ng + g] = qnew[f];
+            for (int f = 0; f < 9; ++f) qnew[f] = rk_combine<R>(rs, qs[f], qb[f], rhs[f], a.dt);
+            if (valid) {
+#pragma unroll
+                for (int f = 0; f < 9; ++f) qout[f * ng + g] = qnew[f];
             }
         }
     }
