@@ -63,6 +63,24 @@ inline std::vector<double> mfma_operator_fragments(const std::vector<double> &D,
     return out;
 }
 
+// [Dr | Ds | Dt] (Np x 3Np) in the A-operand order of the 16x16x4 MFMA, for
+// the volume term of PML elements, which differentiates contravariant fluxes
+// (each column needs sum_o D_o C_o): row tile rt holds nodes 16rt..16rt+15;
+// k-step ks covers operator o = ks / KS at nodes 4(ks % KS)..+3 (KS =
+// ceil(Np/4)); lane l holds node 16rt + l%16 at column 4(ks % KS) + l/16 of
+// operator o (zero beyond Np).  Layout [rt][ks][l].
+inline std::vector<double> mfma_operator_fragments_k(const std::vector<double> &D, int np) {
+    const int nt = (np + 15) / 16, kso = (np + 3) / 4, nk = 3 * kso;
+    std::vector<double> out((size_t)nt * nk * 64, 0.0);
+    for (int rt = 0; rt < nt; ++rt)
+        for (int ks = 0; ks < nk; ++ks)
+            for (int l = 0; l < 64; ++l) {
+                const int op = ks / kso, m = 4 * (ks % kso) + l / 16, node = 16 * rt + l % 16;
+                if (node < np && m < np) out[((size_t)rt * nk + ks) * 64 + l] = D[((size_t)op * np + node) * np + m];
+            }
+    return out;
+}
+
 // LIFT in the A-operand order of the 16x16x4 MFMA: row tile mt holds nodes
 // 16mt..16mt+15, k-step ks face nodes 4ks..4ks+3; lane l holds node
 // 16mt + l%16 at face node 4ks + l/16 (zero beyond Np).  LT is [NFP][NP]

@@ -48,7 +48,7 @@ struct Batch : BatchBase {
     R *u_slot_lo = nullptr;
     R *th[2] = {nullptr, nullptr};
     R *paux[2] = {nullptr, nullptr};
-    R *D = nullptr, *LT = nullptr, *mass = nullptr, *metric = nullptr, *fscale = nullptr, *frame = nullptr;
+    R *D = nullptr, *DK = nullptr, *LT = nullptr, *mass = nullptr, *metric = nullptr, *fscale = nullptr, *frame = nullptr;
     R *jac = nullptr;
     int *fmask = nullptr, *nbr = nullptr, *ftype = nullptr;
     std::vector<int> fmask_host;
@@ -201,7 +201,7 @@ struct Batch : BatchBase {
         a.w23 = f32w ? (R)(2.0f / 3.0f) : (R)(2.0 / 3.0);
         a.pml_compat = (cfg.compat & NEXD_COMPAT_PML_LAST_ELEMENT) ? 1 : 0;
         a.last_flux = last_flux;
-        a.D = D; a.LT = LT; a.fmask = fmask;
+        a.D = D; a.DK = DK; a.LT = LT; a.fmask = fmask;
         a.metric = metric; a.fscale = fscale; a.frame = frame; a.matc = matc;
         a.nbr = nbr; a.ftype = ftype; a.anel = anel;
         for (int i = 0; i < nexd::NMB; ++i) a.wl[i] = (R)wl[i];
@@ -482,7 +482,9 @@ int upload_ops(Batch<R> *b, const double *dr, const double *ds, const double *dt
     const std::string err = nexd::convert_operators(b->np, b->npf, dr, ds, dt, lift, mass, fmask, o);
     if (!err.empty()) return b->fail(NEXD_E_ARG, err);
     int rc;
-    if ((rc = b->upload(&b->D, cvt<R>(nexd::mfma_operator_fragments(o.D, b->np, sizeof(R) == 8)))) || (rc = b->upload(&b->LT, cvt<R>(nexd::mfma_lift_fragments(o.LT, b->np, 4 * b->npf)))) ||
+    if ((rc = b->upload(&b->D, cvt<R>(nexd::mfma_operator_fragments(o.D, b->np, sizeof(R) == 8)))) ||
+        (rc = b->upload(&b->DK, cvt<R>(nexd::mfma_operator_fragments_k(o.D, b->np)))) ||
+        (rc = b->upload(&b->LT, cvt<R>(nexd::mfma_lift_fragments(o.LT, b->np, 4 * b->npf)))) ||
         (rc = b->upload(&b->mass, cvt<R>(o.M))) || (rc = b->upload(&b->fmask, o.fm)))
         return rc;
     b->fmask_host = o.fm;

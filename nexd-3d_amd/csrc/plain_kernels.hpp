@@ -68,6 +68,7 @@ struct StageArgs {
     const R *last_flux;         // [27][NP] fluxes of the last element (compat)
     int pml_compat;
     const R *D;                 // Dr/Ds/Dt as MFMA A-operand fragments [RT][KS][64]
+    const R *DK;                // [Dr|Ds|Dt] as MFMA A-operand fragments [RTV][3KS][64] (PML)
     const R *LT;                // LIFT as MFMA A-operand fragments [MTL][NFP/4][64]
     const int *fmask;           // [4][NPF] 0-based
     const R *metric;            // [9][nglob] rx ry rz sx sy sz tx ty tz
@@ -268,21 +269,47 @@ __device__ __forceinline__ void elastic_flux_node(const R q[9], R ef0, R ef1, R 
 __device__ constexpr int kPmlCols = 24;
 
 // Block shape of one stage kernel instantiation.  Node threads (element-in-
-// block, node) do the loads, the face term, LIFT and the update; for the
-// volume term every wave runs 16x16x4 MFMAs over tiles of the block's
-// columns (q, or the PML-modified fluxes), with the columns' fragments in
-// registers and the operator fragments streamed from L2.
+// block, node) do the loads, the face term, LIFT and the update; the volume
+// and LIFT products run on the matrix cores over all four waves.
+//
+// Volume term.  A = [Dr;Ds;Dt] (row tiles of 4 nodes x 4 operators) times
+// the block's columns, held as fragments in registers: the 9 fields outside
+// the PML; in FP64 PML elements the 24 modified fluxes (F_i for d/dx, G_i
+// for d/dy, H_i for d/dz) and, with attenuation, the 3 velocities.  The
+// metric of each output node is applied to the three derivatives in the
+// epilogue, as the reference does.
+//
+// FP32 PML elements use the contravariant form: on a straight-sided
+// tetrahedron the metric is constant, so the sum of the flux derivatives is
+//   rhs_i = Dr C_r,i + Ds C_s,i + Dt C_t,i,  C_o,i = o_x F_i + o_y G_i + o_z H_i,
+// one product of A = [Dr|Ds|Dt] (Np x 3Np) with 9 columns per element
+// instead of three operators on 24 columns; with attenuation 6 more columns
+// give the strain rates the memory variables use, dv_l/dx_k + dv_k/dx_l =
+// sum_o D_o (o_k v_l + o_l v_k) (one term on the diagonal).  Each wave
+// holds one column tile's values at every k-step in registers and streams
+// the operator.  The metric the reference
+// computes varies between the nodes of an element at the level of single-
+// precision rounding (its geometry is single precision), which this form
+// averages over the element's nodes: a relative change near 1e-7, below the
+// FP32 path's own error but above the FP64 path's parity bound.
 template <int ORDER, bool ATT, bool PML, typename R>
 struct Shape {
     static constexpr int NP = Dims<ORDER>::NP, NPF = Dims<ORDER>::NPF, NFP = Dims<ORDER>::NFP;
-    static constexpr int NCOL = PML ? (ATT ? 27 : 24) : 9;     // columns per element
-    static constexpr int NV = PML ? (ATT ? 33 : 24) : 27;      // volume values per node
+    static constexpr bool KC = PML && sizeof(R) == 4;          // contravariant volume term
+    // volume columns per element
+    static constexpr int NCOL = KC ? (ATT ? 15 : 9) : (PML ? (ATT ? 27 : 24) : 9);
+    // volume values per node in LDS (row stride; odd for the contravariant
+    // form so the node threads' row reads fall in different banks)
+    static constexpr int NV = KC ? (NCOL | 1) : (PML ? (ATT ? 33 : 24) : 27);
     static constexpr int NFC = ATT ? 15 : 9;                    // face values per face node
-    static constexpr int NFCP = ATT ? 16 : 10;                  // face-value rows in LDS
     // LDS row lengths, odd so that threads reading or writing one value of
     // consecutive rows (one row per lane) fall in different banks
     static constexpr int SFN = NFP | 1;                         // face values: row of face nodes
     static constexpr int sln(int eb) { return (eb * NP) | 1; }  // lifted values: row of block nodes
+    // rows of the first LDS region: q, and in the PML the 24 modified fluxes
+    // or, contravariant form, per operator o the 9 fluxes C_o,i and the
+    // metric row o_x o_y o_z (one stride for both kinds of column)
+    static constexpr int ROWS_A = KC ? 9 + 36 : (PML ? 33 : 9);
     // four waves per block: the node phases use EB*NP threads of them, the
     // volume term all four
     static constexpr int NTHR = 256;
@@ -290,48 +317,58 @@ struct Shape {
     // two FP64 blocks or four FP32 blocks, i.e. 2 or 4 waves per SIMD
     static constexpr int BPC = sizeof(R) == 8 ? 2 : 4;
     static constexpr int LDS_BYTES = 160 * 1024 / BPC;
-    // LDS regions, in reals: q and PML flux columns (a, from 0) and face
-    // values (b, after a; [EB][NFCP][SFN]) until the operand fragments are in
-    // registers; then volume values (c, from 0) and the lifted face values
-    // ([16][sln], after max(a, c)); the metric (m) lives from the first phase
-    // to the assembly, after all of them; the energy values and the block
-    // reduction alias everything at the end
+    // LDS regions, in reals:
+    //   a (from 0): q (and the PML fluxes); the volume values [NT][NV]
+    //     reuse it once the columns are in registers or, in the
+    //     contravariant form, after the volume product
+    //   face values [EB][NFC][SFN] after a, until they are in registers;
+    //     the lifted values [16][sln] alias them (contravariant form) or
+    //     follow the largest of a and the volume values
+    //   metric [9][NT] after all of them, from the first phase to the end of
+    //     the volume product (in the first region in the contravariant form)
+    //   the energy values and the block reduction alias everything at the end
+    static constexpr int off_face(int eb) { return ROWS_A * eb * NP; }
     static constexpr int off_lift(int eb) {
         const int nt = eb * NP;
-        const int a = (9 + (PML ? 24 : 0)) * nt;
-        const int c = nt * NV;
-        return a > c ? a : c;
+        if (KC) return off_face(eb);
+        return ROWS_A * nt > nt * NV ? ROWS_A * nt : nt * NV;
     }
     static constexpr int off_metric(int eb) {
-        const int nt = eb * NP;
-        const int ab = (9 + (PML ? 24 : 0)) * nt + eb * NFCP * SFN;
+        const int f = off_face(eb) + eb * NFC * SFN;
         const int l = off_lift(eb) + 16 * sln(eb);
-        return ab > l ? ab : l;
+        return f > l ? f : l;
     }
     static constexpr int lds_reals(int eb) {
-        const int m = off_metric(eb) + 9 * eb * NP;
+        const int m = off_metric(eb) + (KC ? 0 : 9 * eb * NP);
         const int e = 15 * eb * NP + NTHR;
         return m > e ? m : e;
     }
+    // elements per block: at most one node per thread and 16 tiles of
+    // columns, within the LDS share (the element ids take EB ints besides)
     static constexpr int pick_eb() {
         int eb = 256 / NP < 256 / NCOL ? 256 / NP : 256 / NCOL;
         if (eb < 1) eb = 1;
-        while (eb > 1 && lds_reals(eb) * (int)sizeof(R) > LDS_BYTES) --eb;
+        while (eb > 1 && lds_reals(eb) * (int)sizeof(R) + 4 * eb > LDS_BYTES) --eb;
         return eb;
     }
     static constexpr int EB = pick_eb();
     static constexpr int NT = EB * NP;
     static constexpr int NCT = EB * NCOL;
     static constexpr int LDS = lds_reals(EB);
+    static constexpr int OFF_F = off_face(EB);
     static constexpr int OFF_M = off_metric(EB);
     static constexpr int OFF_L = off_lift(EB);
-    // volume product on the matrix cores: [Dr;Ds;Dt] (row tiles of 4 nodes
-    // x 4 operators) times the block's columns (tiles of 16), k-steps of 4
+    static_assert(!KC || NT * NV <= ROWS_A * NT, "volume values must fit the first region");
     static constexpr int NW = NTHR / 64;             // waves
+    static constexpr int KS = (NP + 3) / 4;          // k-steps over one operator's columns
     static constexpr int CTL = (NCT + 15) / 16;      // column tiles
-    static constexpr int CPW = (CTL + NW - 1) / NW;  // column tiles per wave
-    static constexpr int KS = (NP + 3) / 4;          // k-steps
-    static constexpr int RT = (NP + 3) / 4;          // row tiles
+    // [Dr;Ds;Dt]: row tiles of 4 nodes, column tiles per wave
+    static constexpr int CPW = (CTL + NW - 1) / NW;
+    static constexpr int RT = (NP + 3) / 4;
+    // contravariant form: node tiles of 16, k-steps over [Dr|Ds|Dt]; each
+    // wave takes column tiles w, w+NW, ... against all node tiles
+    static constexpr int RTV = (NP + 15) / 16;
+    static constexpr int KSV = 3 * KS;
     // LIFT on the matrix cores: LIFT (node tiles of 16) times each element's
     // face values (16 columns: NFC values and padding), k-steps of 4 face nodes
     static constexpr int MTL = (NP + 15) / 16;       // node tiles
@@ -363,14 +400,16 @@ __global__ void __launch_bounds__((Shape<ORDER, ATT, PML, R>::NTHR), (Shape<ORDE
 stage_kernel(StageArgs<R> a) {
     using S = Shape<ORDER, ATT, PML, R>;
     constexpr int NP = S::NP, NPF = S::NPF, NFP = S::NFP, EB = S::EB, NT = S::NT;
-    constexpr int NCOL = S::NCOL, NV = S::NV, NFC = S::NFC, NFCP = S::NFCP, NTHR = S::NTHR;
+    constexpr int NCOL = S::NCOL, NV = S::NV, NFC = S::NFC, NTHR = S::NTHR;
     constexpr int SFN = S::SFN, SLN = S::sln(S::EB);
     __shared__ __align__(16) R lds[S::LDS];
     __shared__ int sElem[EB];             // the block's element ids, for the face phase
+    constexpr bool KC = S::KC;
     R *sQ = lds;                          // [9][NT]
-    R *sX = lds + 9 * NT;                 // [24][NT]    PML flux columns
-    R *sF = lds + (9 + (PML ? 24 : 0)) * NT;   // [EB][NFCP][SFN]  face values, face-node rows
-    R *sV = lds;                          // [NT][NV]    after the columns are in registers
+    R *sX = lds + 9 * NT;                 // [24][NT]    PML flux columns (FP64)
+    R *sC = lds + 9 * NT;                 // [3][12][NT] C_o,i and o_k, contravariant form (FP32 PML)
+    R *sF = lds + S::OFF_F;               // [EB][NFC][SFN]  face values, face-node rows
+    R *sV = lds;                          // [NT][NV]    volume values, after the volume product
     R *sM = lds + S::OFF_M;               // [9][NT]     metric of the block's nodes
     R *sL = lds + S::OFF_L;               // [16][SLN]   lifted face values, node rows
     constexpr int NRED = NTHR <= 64 ? 64 : (NTHR <= 128 ? 128 : 256);   // power of two >= NTHR
@@ -423,7 +462,7 @@ stage_kernel(StageArgs<R> a) {
     const R *pbase = a.pbase + (size_t)shot * 27 * ng;
     R *pout = a.pout + (size_t)shot * 27 * ng;
     R *sLast = sF;                        // [27][NP] during phase 1
-    static_assert(!PML || 27 * NP <= EB * NFCP * S::SFN, "last-element fluxes must fit the face region");
+    static_assert(!PML || 27 * NP <= EB * NFC * S::SFN, "last-element fluxes must fit the face region");
     if (node_thr) {
         src_first = a.src_head[(size_t)shot * (a.nglob / NP) + e];
         rec_slot_e = a.rec_slot[e];
@@ -460,7 +499,7 @@ stage_kernel(StageArgs<R> a) {
             sQ[f * NT + t] = qs[f];
         }
 #pragma unroll
-        for (int k = 0; k < 9; ++k) sM[k * NT + t] = mv[k];
+        for (int k = 0; k < 9; ++k) (KC ? sC + ((k / 3) * 12 + 9 + k % 3) * NT : sM + k * NT)[t] = mv[k];
         rho = m2;
         ef0 = -(m3 + 2 * m4); ef1 = -m3; ef2 = -m4; ef3 = R(-1.0) / rho;
         if constexpr (PML) {
@@ -470,7 +509,12 @@ stage_kernel(StageArgs<R> a) {
             R Fm[9], Gm[9], Hm[9];
             elastic_flux_node(qs, ef0, ef1, ef2, ef3, Fm, Gm, Hm);
             const RkStage<R> rs = rk_stage<R>(a.scheme, a.irk, a.w13, a.w23);
+            // F_4, G_5 and H_3 are identically zero (src/waveMod.f90:37-62) and
+            // left out of the volume columns.  Contravariant form: C_o,i = o_x
+            // F_i + o_y G_i + o_z H_i, o = r, s, t with the node's metric
+            // (mv = rx ry rz sx sy sz tx ty tz)
             int col = 0;
+            R cf[KC ? 27 : 1];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 R pv[9];
@@ -485,7 +529,16 @@ stage_kernel(StageArgs<R> a) {
                     const int c = d * 9 + i;
                     const R c0 = cur[c];
                     const R mod = (c0 + FF[i]) * rk;
-                    if (i != skip) sX[(col++) * NT + t] = mod;
+                    if constexpr (KC) {
+#pragma unroll
+                        for (int o = 0; o < 3; ++o) {
+                            const R m = mv[3 * o + d];
+                            if (d == 0) cf[o * 9 + i] = i != skip ? m * mod : R(0);
+                            else if (i != skip) cf[o * 9 + i] = cf[o * 9 + i] + m * mod;
+                        }
+                    } else if (i != skip) {
+                        sX[(col++) * NT + t] = mod;
+                    }
                     // read unconditionally and selected (the region is
                     // stale without compat): no branch between components
                     const R lastv = sLast[c * NP + j];
@@ -503,6 +556,10 @@ stage_kernel(StageArgs<R> a) {
                 // fence per direction keeps the working set in registers
                 __builtin_amdgcn_sched_barrier(0);
             }
+            if constexpr (KC) {
+#pragma unroll
+                for (int c = 0; c < 27; ++c) sC[((c / 9) * 12 + c % 9) * NT + t] = cf[c];
+            }
         }
     }
     __syncthreads();
@@ -513,7 +570,7 @@ stage_kernel(StageArgs<R> a) {
         const int fidx = eblk * EB + fb;
         // value f of this face node at dst[f * SFN]: consecutive threads
         // write consecutive words
-        R *dst = sF + fb * NFCP * SFN + k;
+        R *dst = sF + fb * NFC * SFN + k;
         if (fidx >= a.nlist) {
 #pragma unroll
             for (int f = 0; f < NFC; ++f) dst[f * SFN] = 0;
@@ -548,12 +605,8 @@ stage_kernel(StageArgs<R> a) {
     __syncthreads();
 
     // The memory variables and anelastic coefficients of the assembly are
-    // loaded in one batch; FP32 blocks issue that batch before the
-    // matrix-core phase, whose ~20k cycles then hide the round trip (the
-    // barriers wait for LDS traffic only).  FP64 blocks, and FP32 blocks
-    // outside the PML, have no registers to spare and load it at the
-    // assembly, as all do for the RK bases.
-    constexpr bool kEarly = sizeof(R) == 4 && PML;
+    // loaded in one batch at the assembly (loading them before the
+    // matrix-core phase measured neutral and costs registers).
     R th[ATT ? 18 : 1], avc[ATT ? 9 : 1];
     auto load_memory_variables = [&]() {
         if constexpr (ATT) {
@@ -573,32 +626,33 @@ stage_kernel(StageArgs<R> a) {
             }
         }
     };
-    if constexpr (kEarly) load_memory_variables();
 
     // ---- operand fragments into registers ------------------------------------
-    // volume: each wave takes column tiles w, w+NW, ... of the block's
-    // EB*NCOL columns (a q field, a PML flux column or, with PML and
-    // attenuation, a velocity column): B fragment = column 16*tile + l%16 at
-    // node k = 4*ks + l/16
     constexpr int NW = S::NW, CTL = S::CTL, CPW = S::CPW, KS = S::KS, RT = S::RT, NCT = S::NCT;
     constexpr int MTL = S::MTL, KSL = S::KSL, EPW = S::EPW;
     const int lane = t & 63;
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
-    R xb[CPW][KS];
+    // volume with [Dr;Ds;Dt]: each wave takes column tiles w, w+NW, ... of
+    // the block's EB*NCOL columns (a q field, a PML flux column or, with PML
+    // and attenuation, a velocity column): B fragment = column 16*tile + l%16
+    // at node k = 4*ks + l/16
+    R xb[KC ? 1 : CPW][KS];
+    if constexpr (!KC) {
 #pragma unroll
-    for (int c = 0; c < CPW; ++c) {
-        const int col = (w + c * NW) * 16 + (lane & 15);
-        const bool cok = w + c * NW < CTL && col < NCT;
-        const int ceb = cok ? col / NCOL : 0, cc = cok ? col - (col / NCOL) * NCOL : 0;
-        const R *colp;
-        if constexpr (PML) colp = cc < 24 ? sX + cc * NT : sQ + (6 + cc - 24) * NT;
-        else colp = sQ + cc * NT;
-        colp += ceb * NP;
+        for (int c = 0; c < CPW; ++c) {
+            const int col = (w + c * NW) * 16 + (lane & 15);
+            const bool cok = w + c * NW < CTL && col < NCT;
+            const int ceb = cok ? col / NCOL : 0, cc = cok ? col - (col / NCOL) * NCOL : 0;
+            const R *colp;
+            if constexpr (PML) colp = cc < 24 ? sX + cc * NT : sQ + (6 + cc - 24) * NT;
+            else colp = sQ + cc * NT;
+            colp += ceb * NP;
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-            const int m = ks * 4 + (lane >> 4);
-            const R v = colp[m < NP ? m : 0];
-            xb[c][ks] = (cok && m < NP) ? v : R(0);
+            for (int ks = 0; ks < KS; ++ks) {
+                const int m = ks * 4 + (lane >> 4);
+                const R v = colp[m < NP ? m : 0];
+                xb[c][ks] = (cok && m < NP) ? v : R(0);
+            }
         }
     }
     // LIFT: each wave takes elements w, w+NW, ...: B fragment = face value
@@ -609,16 +663,19 @@ stage_kernel(StageArgs<R> a) {
         const int el = w + c * NW;
         const int f = lane & 15;
         const bool ok = el < EB && f < NFC;
-        const R *fp = sF + ((ok ? el : 0) * NFCP + (ok ? f : 0)) * S::SFN + (lane >> 4);
+        const R *fp = sF + ((ok ? el : 0) * NFC + (ok ? f : 0)) * SFN + (lane >> 4);
 #pragma unroll
         for (int ks = 0; ks < KSL; ++ks) {
             const R v = fp[ks * 4];
             fb[c][ks] = ok ? v : R(0);
         }
     }
-    __syncthreads();   // sQ/sX/sF are dead from here; sV and sL reuse them
+    // the face values are dead from here (the contravariant form's lifted
+    // values reuse them); otherwise the columns are in registers and the
+    // volume values reuse their region
+    __syncthreads();
 
-    // ---- LIFT on the matrix cores: sL[node][f] = sum_k LIFT(node,k) F(k,f)
+    // ---- LIFT on the matrix cores: sL[f][node] = sum_k LIFT(node,k) F(k,f)
     // (src/timeloopMod.f90:463-465); the face values already carry fscale/2 --
     {
         using V4 = typename Mfma<R>::v4;
@@ -657,15 +714,104 @@ stage_kernel(StageArgs<R> a) {
         }
     }
 
-    // ---- volume term: [Dr;Ds;Dt] x columns on the matrix cores, then the
-    // metric of each node (src/timeloopMod.f90:407-420) -----------------------
-    {
+    if constexpr (KC) {
+        // ---- volume term, contravariant form: [Dr|Ds|Dt] x contravariant
+        // columns on the matrix cores (src/timeloopMod.f90:399-420) -----------
+        // Column 16*tile + l%16 is (element ce, kind cc): cc < 9 the fluxes
+        // C_o,cc; else strain rate s = cc - 9 (ex ey ez exy eyz exz), whose
+        // column is o_k1 v_c1 (+ o_k2 v_c2 off the diagonal).  At k-step ks
+        // its B value is node 4*(ks % KS) + l/16 of operator block ks / KS.
+        using V4 = typename Mfma<R>::v4;
+        constexpr int RTV = S::RTV, KSV = S::KSV;
+        constexpr int CPK = (CTL + NW - 1) / NW;   // column tiles per wave
+        constexpr int OST = 12 * NT;               // LDS stride between operator blocks
+        constexpr int CH = 3;                      // k-steps per chunk of operator fragments
+        constexpr int NCH = KSV / CH;              // KSV = 3*KS is a multiple
+        V4 acc[CPK][RTV];
+        const R *afr = a.DK + lane;
+#pragma unroll
+        for (int c = 0; c < CPK; ++c) {
+            const int ct = w + c * NW;
+            if (ct >= CTL) break;
+            const int col = ct * 16 + (lane & 15);
+            const bool cok = col < NCT;
+            const int ce = cok ? col / NCOL : 0, cc = cok ? col - (col / NCOL) * NCOL : 0;
+            const bool flux = !ATT || cc < 9;
+            const int st = flux ? 0 : cc - 9;
+            const int k1 = st < 3 ? st : (st == 4 ? 1 : 0), c1 = st < 3 ? st : (st == 3 ? 1 : 2);
+            const int k2 = st == 3 ? 1 : 2, c2 = st == 4 ? 1 : 0;
+            const bool two = st >= 3;
+            const int r1 = 9 * NT + (flux ? cc : 9 + k1) * NT + ce * NP, q1 = (6 + c1) * NT + ce * NP;
+            const int r2 = 9 * NT + (9 + k2) * NT + ce * NP, q2 = (6 + c2) * NT + ce * NP;
+            // the lane's column at every k-step
+            R bf[KSV];
+#pragma unroll
+            for (int ks = 0; ks < KSV; ++ks) {
+                const int o = ks / KS, m = 4 * (ks - o * KS) + (lane >> 4);
+                const int mm = m < NP ? m : 0;   // the operator is zero beyond Np
+                const R x1 = lds[r1 + o * OST + mm];
+                if constexpr (ATT) {
+                    const R y1 = lds[q1 + mm], x2 = lds[r2 + o * OST + mm], y2 = lds[q2 + mm];
+                    const R e = x1 * y1 + (two ? x2 * y2 : R(0));
+                    bf[ks] = flux ? x1 : e;
+                } else {
+                    bf[ks] = x1;
+                }
+            }
+#pragma unroll
+            for (int rt = 0; rt < RTV; ++rt) acc[c][rt] = V4{0, 0, 0, 0};
+            // operator fragments of every node tile in chunks of CH k-steps,
+            // the next chunk loading while this one computes
+            R abuf[2][CH][RTV];
+#pragma unroll
+            for (int h = 0; h < CH; ++h)
+#pragma unroll
+                for (int rt = 0; rt < RTV; ++rt) abuf[0][h][rt] = afr[(rt * KSV + h) * 64];
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) {
+                if (ch + 1 < NCH) {
+#pragma unroll
+                    for (int h = 0; h < CH; ++h)
+#pragma unroll
+                        for (int rt = 0; rt < RTV; ++rt)
+                            abuf[(ch + 1) & 1][h][rt] = afr[(rt * KSV + (ch + 1) * CH + h) * 64];
+                }
+#pragma unroll
+                for (int h = 0; h < CH; ++h)
+#pragma unroll
+                    for (int rt = 0; rt < RTV; ++rt)
+                        acc[c][rt] = Mfma<R>::fma(abuf[ch & 1][h][rt], bf[ch * CH + h], acc[c][rt]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        __syncthreads();   // q, the fluxes and the metric are dead: sV reuses them
+        // accumulator element v: node row 4*(l/16)+v (FP32) or 4v + l/16
+        // (FP64) of node tile rt, column l%16 of the column tile
+#pragma unroll
+        for (int c = 0; c < CPK; ++c) {
+            const int ct = w + c * NW;
+            if (ct >= CTL) break;
+            const int col = ct * 16 + (lane & 15);
+            if (col < NCT) {
+                const int ce = col / NCOL, cc = col - ce * NCOL;
+#pragma unroll
+                for (int rt = 0; rt < RTV; ++rt)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int node = rt * 16 + (sizeof(R) == 8 ? 4 * v + (lane >> 4) : 4 * (lane >> 4) + v);
+                        if (node < NP) sV[(ce * NP + node) * NV + cc] = acc[c][rt][v];
+                    }
+            }
+        }
+    } else {
+        // ---- volume term: [Dr;Ds;Dt] x columns on the matrix cores, then the
+        // metric of each node (src/timeloopMod.f90:407-420) --------------------
         using V4 = typename Mfma<R>::v4;
         const R *dfr = a.D + lane;
         // operator fragments double-buffered: row tile rt+1 loads while rt
         // computes (vector loads return in order); two tiles per iteration so
         // the buffers swap roles without copies
-        R fa[KS], fb[KS];
+        R fa[KS], fb2[KS];
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) fa[ks] = dfr[ks * 64];
 #pragma unroll 1
@@ -674,8 +820,8 @@ stage_kernel(StageArgs<R> a) {
             for (int h = 0; h < 2; ++h) {
                 const int rt = rt0 + h;
                 if (rt >= RT) break;
-                R *af = h == 0 ? fa : fb;
-                R *an = h == 0 ? fb : fa;
+                R *af = h == 0 ? fa : fb2;
+                R *an = h == 0 ? fb2 : fa;
                 const int rn = rt + 1 < RT ? rt + 1 : rt;   // unconditional: no branch
 #pragma unroll
                 for (int ks = 0; ks < KS; ++ks) an[ks] = dfr[(rn * KS + ks) * 64];
@@ -718,7 +864,7 @@ stage_kernel(StageArgs<R> a) {
     __syncthreads();
 
     // ---- assemble the right-hand side (node threads) -----------------------
-    if constexpr (!kEarly) load_memory_variables();
+    load_memory_variables();
     R thb[ATT ? 18 : 1], qb[9];
     if (node_thr) {
         if constexpr (ATT) {
@@ -752,6 +898,7 @@ stage_kernel(StageArgs<R> a) {
     }
     R rhs[9];
     R gv[9];   // velocity gradients: gv[c*3+k] = d v_c / d x_k
+    R sr[6];   // strain rates ex ey ez exy eyz exz (contravariant form)
     if (node_thr) {
         const R *v = sV + t * NV;
         if constexpr (!PML) {
@@ -770,16 +917,25 @@ stage_kernel(StageArgs<R> a) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) { gv[c * 3] = dx[6 + c]; gv[c * 3 + 1] = dy[6 + c]; gv[c * 3 + 2] = dz[6 + c]; }
         } else {
+            if constexpr (KC) {
 #pragma unroll
-            for (int i = 0; i < 9; ++i) {
-                const R vF = i == 4 ? R(0) : v[i < 4 ? i : i - 1];
-                const R vG = i == 5 ? R(0) : v[8 + (i < 5 ? i : i - 1)];
-                const R vH = i == 3 ? R(0) : v[16 + (i < 3 ? i : i - 1)];
-                rhs[i] = vF + vG + vH;
-            }
-            if constexpr (ATT) {
+                for (int i = 0; i < 9; ++i) rhs[i] = v[i];
+                if constexpr (ATT) {
 #pragma unroll
-                for (int c = 0; c < 9; ++c) gv[c] = v[24 + c];
+                    for (int c = 0; c < 6; ++c) sr[c] = v[9 + c];
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 9; ++i) {
+                    const R vF = i == 4 ? R(0) : v[i < 4 ? i : i - 1];
+                    const R vG = i == 5 ? R(0) : v[8 + (i < 5 ? i : i - 1)];
+                    const R vH = i == 3 ? R(0) : v[16 + (i < 3 ? i : i - 1)];
+                    rhs[i] = vF + vG + vH;
+                }
+                if constexpr (ATT) {
+#pragma unroll
+                    for (int c = 0; c < 9; ++c) gv[c] = v[24 + c];
+                }
             }
         }
     }
@@ -787,8 +943,13 @@ stage_kernel(StageArgs<R> a) {
     // ---- attenuation (src/timeloopMod.f90:426-461), one mechanism at a time --
     if constexpr (ATT) {
         if (node_thr) {
-            const R ex = gv[0], ey = gv[4], ez = gv[8];
-            const R exy = gv[3] + gv[1], eyz = gv[7] + gv[5], exz = gv[6] + gv[2];
+            R ex, ey, ez, exy, eyz, exz;
+            if constexpr (KC) {
+                ex = sr[0]; ey = sr[1]; ez = sr[2]; exy = sr[3]; eyz = sr[4]; exz = sr[5];
+            } else {
+                ex = gv[0]; ey = gv[4]; ez = gv[8];
+                exy = gv[3] + gv[1]; eyz = gv[7] + gv[5]; exz = gv[6] + gv[2];
+            }
             R *thout = a.thout + (size_t)shot * 18 * ng;
             const RkStage<R> rs = rk_stage<R>(a.scheme, a.irk, a.w13, a.w23);
 #pragma unroll
