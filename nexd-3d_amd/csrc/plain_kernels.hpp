@@ -440,7 +440,35 @@ stage_kernel(StageArgs<R> a) {
     const int eb = node_thr ? t / NP : 0, j = node_thr ? t - (t / NP) * NP : 0;
     const int idx = eblk * EB + eb;
     const bool valid = node_thr && idx < a.nlist;
+    // The last element's fluxes (PML, compat), the same for every element,
+    // are staged once per block in the face-value region of LDS, which is
+    // free until the face phase; they are loaded together with the element
+    // id, so their round trip overlaps that one instead of following the
+    // phase-1 batch.
+    R *sLast = lds + S::OFF_F;            // [27][NP] during phase 1
+    static_assert(!PML || 27 * NP <= EB * NFC * S::SFN, "last-element fluxes must fit the face region");
+    constexpr int NLS = PML ? (27 * NP + NTHR - 1) / NTHR : 1;
+    R lst[NLS];
+    if constexpr (PML) {
+        if (a.pml_compat) {
+            const R *lastf = a.last_flux + (size_t)shot * 27 * NP;
+#pragma unroll
+            for (int r = 0; r < NLS; ++r) {
+                const int i = t + r * NTHR;
+                lst[r] = lastf[i < 27 * NP ? i : 0];
+            }
+        }
+    }
     const int e = valid ? a.elist[idx] : 0;
+    if constexpr (PML) {
+        if (a.pml_compat) {
+#pragma unroll
+            for (int r = 0; r < NLS; ++r) {
+                const int i = t + r * NTHR;
+                if (i < 27 * NP) sLast[i] = lst[r];
+            }
+        }
+    }
     const size_t g = (size_t)e * NP + j;
     if (node_thr && j == 0) sElem[eb] = e;
     // first source of the element and its receiver slot, loaded with the
@@ -451,9 +479,7 @@ stage_kernel(StageArgs<R> a) {
     // and material and (PML) the coefficients and all 27 auxiliaries and RK
     // bases.  No load is guarded by a branch: padding threads read element
     // 0, and in stage 1 the RK base buffer is the input buffer (the value is
-    // not used).  The last element's fluxes, the same for every element,
-    // are staged once per block in the face-value region of LDS, which is
-    // free until the face phase.
+    // not used).
     R qs[9];
     R ef0 = 0, ef1 = 0, ef2 = 0, ef3 = 0, rho = 1;
     R qv[9], mv[9], m2 = 1, m3 = 0, m4 = 0;
@@ -461,8 +487,6 @@ stage_kernel(StageArgs<R> a) {
     const R *pin = a.pin + (size_t)shot * 27 * ng;
     const R *pbase = a.pbase + (size_t)shot * 27 * ng;
     R *pout = a.pout + (size_t)shot * 27 * ng;
-    R *sLast = sF;                        // [27][NP] during phase 1
-    static_assert(!PML || 27 * NP <= EB * NFC * S::SFN, "last-element fluxes must fit the face region");
     if (node_thr) {
         src_first = a.src_head[(size_t)shot * (a.nglob / NP) + e];
         rec_slot_e = a.rec_slot[e];
@@ -482,13 +506,7 @@ stage_kernel(StageArgs<R> a) {
             for (int c = 0; c < 27; ++c) bse[c] = pbase[c * ng + g];
         }
     }
-    if constexpr (PML) {
-        if (a.pml_compat) {
-            const R *lastf = a.last_flux + (size_t)shot * 27 * NP;
-            for (int i = t; i < 27 * NP; i += NTHR) sLast[i] = lastf[i];
-        }
-        __syncthreads();
-    }
+    if constexpr (PML) __syncthreads();   // the staged fluxes are in LDS
     // Node threads past the end of the list (last block only) carry element
     // 0's values through the node phases: finite, confined to their own
     // columns, never stored.
