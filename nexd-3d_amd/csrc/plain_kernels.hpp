@@ -333,8 +333,12 @@ struct Shape {
         if (KC) return off_face(eb);
         return ROWS_A * nt > nt * NV ? ROWS_A * nt : nt * NV;
     }
+    // the face region also holds the last element's 27 fluxes in phase 1
+    static constexpr int face_reals(int eb) {
+        return eb * NFC * SFN > (PML ? 27 * NP : 0) ? eb * NFC * SFN : 27 * NP;
+    }
     static constexpr int off_metric(int eb) {
-        const int f = off_face(eb) + eb * NFC * SFN;
+        const int f = off_face(eb) + face_reals(eb);
         const int l = off_lift(eb) + 16 * sln(eb);
         return f > l ? f : l;
     }
@@ -346,7 +350,7 @@ struct Shape {
     // elements per block: at most one node per thread and 16 tiles of
     // columns, within the LDS share (the element ids take EB ints besides)
     static constexpr int pick_eb() {
-        int eb = 256 / NP < 256 / NCOL ? 256 / NP : 256 / NCOL;
+        int eb = NTHR / NP < 256 / NCOL ? NTHR / NP : 256 / NCOL;
         if (eb < 1) eb = 1;
         while (eb > 1 && lds_reals(eb) * (int)sizeof(R) + 4 * eb > LDS_BYTES) --eb;
         return eb;
@@ -446,7 +450,7 @@ stage_kernel(StageArgs<R> a) {
     // id, so their round trip overlaps that one instead of following the
     // phase-1 batch.
     R *sLast = lds + S::OFF_F;            // [27][NP] during phase 1
-    static_assert(!PML || 27 * NP <= EB * NFC * S::SFN, "last-element fluxes must fit the face region");
+    static_assert(!PML || 27 * NP <= S::face_reals(EB), "last-element fluxes must fit the face region");
     constexpr int NLS = PML ? (27 * NP + NTHR - 1) / NTHR : 1;
     R lst[NLS];
     if constexpr (PML) {
