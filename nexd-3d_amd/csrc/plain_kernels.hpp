@@ -176,7 +176,10 @@ __device__ __forceinline__ void upwind_flux(const R *fr, R vp, R vs, R rho, R la
 // node's independent updates into separate basic blocks, which the compiler
 // then cannot interleave, and each update's latency would add up.  dt*rhs
 // and a + dt*rhs are rounded separately; the weighted sum is one FMA (exact
-// for stage 1: 0 * base + x).
+// for stage 1: 0 * base + x).  WEIGHTED takes the weighted form for TVD
+// RK2 stage 2 as well (1/2 base + 1/2 (a + dt*rhs), equal up to rounding),
+// which saves four operations per value; the FP32 memory variables and
+// fields use it.
 template <typename R>
 struct RkStage {
     R wb, wx;    // wb * base + wx * (a + dt*rhs)
@@ -189,13 +192,14 @@ __device__ __forceinline__ RkStage<R> rk_stage(int scheme, int irk, R w13, R w23
     if (irk == 2) return {R(0.75), R(0.25), false};
     return {w13, w23, false};
 }
-template <typename R>
+template <typename R, bool WEIGHTED = false>
 __device__ __forceinline__ R rk_combine(const RkStage<R> &s, R a, R base, R rhs, R dt) {
 #pragma clang fp contract(off)
     const R k = rhs * dt;
     const R x = a + k;
-    const R y = ((base + a) + k) * R(0.5);
     const R z = __builtin_fma(s.wb, base, s.wx * x);
+    if constexpr (WEIGHTED) return z;
+    const R y = ((base + a) + k) * R(0.5);
     return s.avg ? y : z;
 }
 
@@ -991,7 +995,7 @@ stage_kernel(StageArgs<R> a) {
                 for (int c = 0; c < 6; ++c) {
                     const int k = i * 6 + c;
                     const R arq = vol[c] + w * lf[9 + c] - w * th[k];
-                    th[k] = rk_combine<R>(rs, th[k], thb[k], arq, a.dt);
+                    th[k] = rk_combine<R, sizeof(R) == 4>(rs, th[k], thb[k], arq, a.dt);
                 }
             }
             // one guarded block of stores
@@ -1055,7 +1059,7 @@ stage_kernel(StageArgs<R> a) {
         } else {
             const RkStage<R> rs = rk_stage<R>(a.scheme, a.irk, a.w13, a.w23);
 #pragma unroll
-            for (int f = 0; f < 9; ++f) qnew[f] = rk_combine<R>(rs, qs[f], qb[f], rhs[f], a.dt);
+            for (int f = 0; f < 9; ++f) qnew[f] = rk_combine<R, sizeof(R) == 4>(rs, qs[f], qb[f], rhs[f], a.dt);
             if (valid) {
 #pragma unroll
                 for (int f = 0; f < 9; ++f) qout[f * ng + g] = qnew[f];
