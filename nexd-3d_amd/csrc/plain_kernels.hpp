@@ -681,12 +681,13 @@ stage_kernel(StageArgs<R> a) {
             }
         }
     }
-    // LIFT: each wave takes elements w, w+NW, ...: B fragment = face value
-    // l%16 (of NFC) at face node 4*ks + l/16
+    // LIFT: wave w takes elements NW-1-w, 2NW-1-w, ... (the last waves have
+    // the fewest volume tiles): B fragment = face value l%16 (of NFC) at face
+    // node 4*ks + l/16
     R fb[EPW][KSL];
 #pragma unroll
     for (int c = 0; c < EPW; ++c) {
-        const int el = w + c * NW;
+        const int el = (NW - 1 - w) + c * NW;
         const int f = lane & 15;
         const bool ok = el < EB && f < NFC;
         const R *fp = sF + ((ok ? el : 0) * NFC + (ok ? f : 0)) * SFN + (lane >> 4);
@@ -708,7 +709,7 @@ stage_kernel(StageArgs<R> a) {
         const R *lfr = a.LT + lane;
 #pragma unroll
         for (int c = 0; c < EPW; ++c) {
-            const int el = w + c * NW;
+            const int el = (NW - 1 - w) + c * NW;
             if (el >= EB) break;
             // LIFT fragments of node tile mt+1 load while mt computes
             R la[KSL], lb[KSL];
