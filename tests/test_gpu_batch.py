@@ -126,14 +126,15 @@ def test_fp32_compensated_rk3():
         assert err.max() <= 1e-4
 
 
-@pytest.mark.parametrize("order", [2, 5])
-def test_fp32_batch_other_orders(order):
-    """FP32 batches at N=2 and N=5 (other block shapes and tilings) against
-    FP64 single shots, attenuation and PML."""
+@pytest.mark.parametrize("order,attenuation", [(1, True), (2, True), (3, True), (5, True), (3, False)])
+def test_fp32_batch_other_orders(order, attenuation):
+    """FP32 batches at other orders (other block shapes, tilings and, in the
+    PML, the contravariant volume term's node tiles and k-step chunks)
+    against FP64 single shots, with PML, with and without attenuation."""
     sb = batch.layered_shots(2, nx=3, ny=3, nz=3, order=order, nt=80, pml_cells=1, nrec_side=2,
-                             full_source_arrays=True)
+                             attenuation=attenuation, full_source_arrays=True)
     got = batch_run(sb, capi.NEXD_FP32)
     for k, s in enumerate(sb.shots):
         err = rel_l2(got[k], fp64_single(s))
-        print(f"  N={order} shot {k}: FP32 {err.max():.3e}", flush=True)
+        print(f"  N={order} att={attenuation} shot {k}: FP32 {err.max():.3e}", flush=True)
         assert err.max() <= 1e-4
