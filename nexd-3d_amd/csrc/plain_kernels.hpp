@@ -753,6 +753,7 @@ stage_kernel(StageArgs<R> a) {
         constexpr int CPK = (CTL + NW - 1) / NW;   // column tiles per wave
         constexpr int OST = 12 * NT;               // LDS stride between operator blocks
         constexpr int CH = 3;                      // k-steps per chunk of operator fragments
+        constexpr int PREF = RTV <= 3 ? 3 : 2;    // chunks in flight (registers allow 3 up to 3 node tiles)
         constexpr int NCH = KSV / CH;              // KSV = 3*KS is a multiple
         V4 acc[CPK][RTV];
         const R *afr = a.DK + lane;
@@ -789,25 +790,29 @@ stage_kernel(StageArgs<R> a) {
             for (int rt = 0; rt < RTV; ++rt) acc[c][rt] = V4{0, 0, 0, 0};
             // operator fragments of every node tile in chunks of CH k-steps,
             // the next chunk loading while this one computes
-            R abuf[2][CH][RTV];
+            constexpr int NB = PREF;   // chunks in flight
+            R abuf[NB][CH][RTV];
 #pragma unroll
-            for (int h = 0; h < CH; ++h)
+            for (int p = 0; p < NB - 1; ++p)
 #pragma unroll
-                for (int rt = 0; rt < RTV; ++rt) abuf[0][h][rt] = afr[(rt * KSV + h) * 64];
+                for (int h = 0; h < CH; ++h)
+#pragma unroll
+                    for (int rt = 0; rt < RTV; ++rt)
+                        abuf[p][h][rt] = afr[(rt * KSV + (p < NCH ? p : 0) * CH + h) * 64];
 #pragma unroll
             for (int ch = 0; ch < NCH; ++ch) {
-                if (ch + 1 < NCH) {
+                if (ch + NB - 1 < NCH) {
 #pragma unroll
                     for (int h = 0; h < CH; ++h)
 #pragma unroll
                         for (int rt = 0; rt < RTV; ++rt)
-                            abuf[(ch + 1) & 1][h][rt] = afr[(rt * KSV + (ch + 1) * CH + h) * 64];
+                            abuf[(ch + NB - 1) % NB][h][rt] = afr[(rt * KSV + (ch + NB - 1) * CH + h) * 64];
                 }
 #pragma unroll
                 for (int h = 0; h < CH; ++h)
 #pragma unroll
                     for (int rt = 0; rt < RTV; ++rt)
-                        acc[c][rt] = Mfma<R>::fma(abuf[ch & 1][h][rt], bf[ch * CH + h], acc[c][rt]);
+                        acc[c][rt] = Mfma<R>::fma(abuf[ch % NB][h][rt], bf[ch * CH + h], acc[c][rt]);
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
