@@ -321,6 +321,9 @@ struct Shape {
     // two FP64 blocks or four FP32 blocks, i.e. 2 or 4 waves per SIMD
     static constexpr int BPC = sizeof(R) == 8 ? 2 : 4;
     static constexpr int LDS_BYTES = 160 * 1024 / BPC;
+    // waves per SIMD at that occupancy (4 SIMDs per CU): the second
+    // __launch_bounds__ argument, which sets the register budget
+    static constexpr int WPS = BPC * (NTHR / 64) / 4;
     // LDS regions, in reals:
     //   a (from 0): q (and the PML fluxes); the volume values [NT][NV]
     //     reuse it once the columns are in registers or, in the
@@ -404,7 +407,7 @@ struct Mfma<double> {
 };
 
 template <int ORDER, bool ATT, bool PML, typename R>
-__global__ void __launch_bounds__((Shape<ORDER, ATT, PML, R>::NTHR), (Shape<ORDER, ATT, PML, R>::BPC))
+__global__ void __launch_bounds__((Shape<ORDER, ATT, PML, R>::NTHR), (Shape<ORDER, ATT, PML, R>::WPS))
 stage_kernel(StageArgs<R> a) {
     using S = Shape<ORDER, ATT, PML, R>;
     constexpr int NP = S::NP, NPF = S::NPF, NFP = S::NFP, EB = S::EB, NT = S::NT;
