@@ -69,6 +69,15 @@ def test_shot_independent_of_batch(cube_shots):
     assert np.array_equal(full[1], alone[0])
 
 
+def test_shot_independent_of_batch_pml(layered_shots):
+    """The same with attenuation and PML (the FP32 contravariant volume
+    term): a shot's seismograms do not depend on the other shots."""
+    full = batch_run(layered_shots, capi.NEXD_FP32)
+    one = batch.ShotBatch(layered_shots.base, [layered_shots.shots[1]], layered_shots.shared_material)
+    alone = batch_run(one, capi.NEXD_FP32)
+    assert np.array_equal(full[1], alone[0])
+
+
 @pytest.mark.parametrize("which", ["cube", "layered"])
 def test_fp32_batch_error_vs_fp64(which, cube_shots, layered_shots):
     sb = cube_shots if which == "cube" else layered_shots
