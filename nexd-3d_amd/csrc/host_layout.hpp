@@ -41,24 +41,25 @@ inline std::string convert_operators(int np, int npf, const double *dr, const do
 }
 
 // Dr, Ds, Dt in the A-operand order of the 16x16x4 MFMA (row tile rt,
-// k-step ks, lane l -> [rt][ks][l]).  Row tile rt holds nodes 4rt..4rt+3
-// and the three operators plus a zero row; lane l of k-step ks holds tile
-// row l%16 at column 4ks + l/16 (zero beyond Np).  The row order inside a
-// tile makes accumulator element v of every lane the operator-v derivative
-// of one node: node-major (4*node + op) for the FP32 instruction, whose
-// lane holds tile rows 4*(l/16)+v, and operator-major (4*op + node) for the
-// FP64 one, whose lane holds rows 4v + l/16.
+// k-step ks, lane l -> [rt][ks][l]); lane l of k-step ks holds tile row
+// l%16 at column 4ks + l/16 (zero beyond Np).  Accumulator element v of a
+// lane in lane group g = l/16 is tile row 4v + g (FP64 instruction) or
+// 4g + v (FP32).  The rows are packed so that each lane group owns nodes
+// g, g+4, g+8, ... with their three operator rows in order: element v of
+// row tile rt is row q = 4rt + v of the group, node g + 4*(q/3), operator
+// q%3.  A lane thus receives the three derivatives of each of its nodes,
+// across consecutive tiles, with no padding row (3*ceil(Np/4) rows per
+// group in ceil(3*ceil(Np/4)/4) tiles).
 inline std::vector<double> mfma_operator_fragments(const std::vector<double> &D, int np, bool f64) {
-    const int nt = (np + 3) / 4, nk = (np + 3) / 4;
+    const int npg = (np + 3) / 4, nt = (3 * npg + 3) / 4, nk = (np + 3) / 4;
     std::vector<double> out((size_t)nt * nk * 64, 0.0);
     for (int rt = 0; rt < nt; ++rt)
         for (int ks = 0; ks < nk; ++ks)
             for (int l = 0; l < 64; ++l) {
                 const int i = l % 16, m = 4 * ks + l / 16;
-                const int node = f64 ? i % 4 : i / 4, op = f64 ? i / 4 : i % 4;
-                const int jr = 4 * rt + node;
-                if (op < 3 && jr < np && m < np)
-                    out[((size_t)rt * nk + ks) * 64 + l] = D[((size_t)op * np + jr) * np + m];
+                const int g = f64 ? i % 4 : i / 4, v = f64 ? i / 4 : i % 4;
+                const int q = 4 * rt + v, node = g + 4 * (q / 3), op = q % 3;
+                if (node < np && m < np) out[((size_t)rt * nk + ks) * 64 + l] = D[((size_t)op * np + node) * np + m];
             }
     return out;
 }

@@ -276,8 +276,8 @@ __device__ constexpr int kPmlCols = 24;
 // block, node) do the loads, the face term, LIFT and the update; the volume
 // and LIFT products run on the matrix cores over all four waves.
 //
-// Volume term.  A = [Dr;Ds;Dt] (row tiles of 4 nodes x 4 operators) times
-// the block's columns, held as fragments in registers: the 9 fields outside
+// Volume term.  A = [Dr;Ds;Dt] (row tiles of (node, operator) rows, no
+// padding row) times the block's columns, held as fragments in registers: the 9 fields outside
 // the PML; in FP64 PML elements the 24 modified fluxes (F_i for d/dx, G_i
 // for d/dy, H_i for d/dz) and, with attenuation, the 3 velocities.  The
 // metric of each output node is applied to the three derivatives in the
@@ -373,9 +373,10 @@ struct Shape {
     static constexpr int NW = NTHR / 64;             // waves
     static constexpr int KS = (NP + 3) / 4;          // k-steps over one operator's columns
     static constexpr int CTL = (NCT + 15) / 16;      // column tiles
-    // [Dr;Ds;Dt]: row tiles of 4 nodes, column tiles per wave
+    // [Dr;Ds;Dt]: column tiles per wave; row tiles packed so that lane
+    // group g (of 4) owns nodes g, g+4, ... with three operator rows each
     static constexpr int CPW = (CTL + NW - 1) / NW;
-    static constexpr int RT = (NP + 3) / 4;
+    static constexpr int RT = (3 * ((NP + 3) / 4) + 3) / 4;
     // contravariant form: node tiles of 16, k-steps over [Dr|Ds|Dt]; each
     // wave takes column tiles w, w+NW, ... against all node tiles
     static constexpr int RTV = (NP + 15) / 16;
@@ -841,38 +842,45 @@ stage_kernel(StageArgs<R> a) {
     } else {
         // ---- volume term: [Dr;Ds;Dt] x columns on the matrix cores, then the
         // metric of each node (src/timeloopMod.f90:407-420) --------------------
+        // Accumulator element v of row tile rt is row q = 4rt + v of the
+        // lane's group g: node g + 4*(q/3), operator q%3.  A node's three
+        // derivatives arrive in order, possibly across two tiles; the metric
+        // is applied when the third arrives.  The tiles are unrolled so that
+        // this bookkeeping is static.
         using V4 = typename Mfma<R>::v4;
         const R *dfr = a.D + lane;
+        const int lg = lane >> 4;
         // operator fragments double-buffered: row tile rt+1 loads while rt
-        // computes (vector loads return in order); two tiles per iteration so
-        // the buffers swap roles without copies
+        // computes (vector loads return in order)
         R fa[KS], fb2[KS];
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) fa[ks] = dfr[ks * 64];
-#pragma unroll 1
-        for (int rt0 = 0; rt0 < RT; rt0 += 2) {
+        R part[CPW][3];
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int rt = rt0 + h;
-                if (rt >= RT) break;
-                R *af = h == 0 ? fa : fb2;
-                R *an = h == 0 ? fb2 : fa;
-                const int rn = rt + 1 < RT ? rt + 1 : rt;   // unconditional: no branch
+        for (int rt = 0; rt < RT; ++rt) {
+            R *af = (rt & 1) ? fb2 : fa;
+            R *an = (rt & 1) ? fa : fb2;
+            if (rt + 1 < RT) {
 #pragma unroll
-                for (int ks = 0; ks < KS; ++ks) an[ks] = dfr[(rn * KS + ks) * 64];
-                const int jr = rt * 4 + (lane >> 4);
+                for (int ks = 0; ks < KS; ++ks) an[ks] = dfr[((rt + 1) * KS + ks) * 64];
+            }
 #pragma unroll
-                for (int c = 0; c < CPW; ++c) {
-                    if (w + c * NW >= CTL) break;
-                    V4 acc = {0, 0, 0, 0};
+            for (int c = 0; c < CPW; ++c) {
+                if (w + c * NW >= CTL) break;
+                V4 acc = {0, 0, 0, 0};
 #pragma unroll
-                    for (int ks = 0; ks < KS; ++ks) acc = Mfma<R>::fma(af[ks], xb[c][ks], acc);
-                    // element v of the accumulator: operator v (Dr, Ds, Dt) of
-                    // node jr, column 16*tile + l%16
-                    const int col = (w + c * NW) * 16 + (lane & 15);
+                for (int ks = 0; ks < KS; ++ks) acc = Mfma<R>::fma(af[ks], xb[c][ks], acc);
+                const int col = (w + c * NW) * 16 + (lane & 15);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int q = 4 * rt + v, op = q % 3;
+                    part[c][op] = acc[v];
+                    if (op != 2) continue;
+                    // column 16*tile + l%16, node jr: its three derivatives
+                    const int jr = lg + 4 * (q / 3);
                     if (col < NCT && jr < NP) {
                         const int ceb = col / NCOL, cc = col - ceb * NCOL;
-                        const R dr = acc[0], ds = acc[1], dt = acc[2];
+                        const R dr = part[c][0], ds = part[c][1], dt = part[c][2];
                         // which metric row(s) the column needs: PML flux columns
                         // one direction, q / velocity columns all three
                         const int kind = PML ? (cc < 8 ? 0 : (cc < 16 ? 1 : (cc < 24 ? 2 : 3))) : 3;
@@ -894,6 +902,7 @@ stage_kernel(StageArgs<R> a) {
                     }
                 }
             }
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
     __syncthreads();
