@@ -44,7 +44,7 @@ struct nexd_ctx {
     double *paux[2] = {nullptr, nullptr};
     double *u_full = nullptr;
     // operators and mesh
-    double *D = nullptr, *DK = nullptr, *LT = nullptr, *mass = nullptr;
+    double *D = nullptr, *LT = nullptr, *mass = nullptr;
     int *fmask = nullptr;
     std::vector<int> fmask_host;
     double *metric = nullptr, *fscale = nullptr, *frame = nullptr, *jac = nullptr;
@@ -220,7 +220,7 @@ StageArgs<double> base_args(nexd_ctx *c) {
     a.w23 = f32w ? (double)(2.0f / 3.0f) : 2.0 / 3.0;
     a.pml_compat = (c->cfg.compat & NEXD_COMPAT_PML_LAST_ELEMENT) ? 1 : 0;
     a.last_flux = c->last_flux;
-    a.D = c->D; a.DK = c->DK; a.LT = c->LT; a.fmask = c->fmask;
+    a.D = c->D; a.DK = nullptr; a.LT = c->LT; a.fmask = c->fmask;   // FP64: no contravariant form
     a.metric = c->metric; a.fscale = c->fscale; a.frame = c->frame; a.matc = c->matc;
     a.nbr = c->nbr; a.ftype = c->ftype; a.anel = c->anel;
     for (int i = 0; i < nexd::NMB; ++i) a.wl[i] = c->wl[i];
@@ -389,7 +389,6 @@ int nexd_upload_operators(nexd_ctx *c, const double *dr, const double *ds, const
     if (!err.empty()) return fail(c, NEXD_E_ARG, err);
     int rc;
     if ((rc = upload(c, &c->D, nexd::mfma_operator_fragments(o.D, c->np, true))) ||
-        (rc = upload(c, &c->DK, nexd::mfma_operator_fragments_k(o.D, c->np))) ||
         (rc = upload(c, &c->LT, nexd::mfma_lift_fragments(o.LT, c->np, 4 * c->npf))) || (rc = upload(c, &c->mass, o.M)) ||
         (rc = upload(c, &c->fmask, o.fm)))
         return rc;
