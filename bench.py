@@ -284,7 +284,7 @@ def main():
         print(json.dumps(summ, indent=1))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "the engine runs on one GPU (multi-GPU is a design, not built); --gpus must be 1"}))
         sys.exit(2)
     if args.uneven:
         uneven_main(args)
