@@ -594,6 +594,13 @@ stage_kernel(StageArgs<R> a) {
     }
     __syncthreads();
 
+    // LIFT fragments of node tile 0 (operator data, from L2): issued before
+    // the face term, whose latency covers theirs
+    const R *lfr = a.LT + (t & 63);
+    R lfa[S::KSL];
+#pragma unroll
+    for (int ks = 0; ks < S::KSL; ++ks) lfa[ks] = lfr[ks * 64];
+
     // ---- face term: upwind flux per face node into sF ----------------------
     for (int it = t; it < EB * NFP; it += NTHR) {
         const int fb = it / NFP, k = it - fb * NFP;
@@ -708,38 +715,51 @@ stage_kernel(StageArgs<R> a) {
 
     // ---- LIFT on the matrix cores: sL[f][node] = sum_k LIFT(node,k) F(k,f)
     // (src/timeloopMod.f90:463-465); the face values already carry fscale/2 --
+    R vfa[KC ? 1 : KS];   // volume operator fragments of row tile 0
+    // contravariant form: the leading chunks (3 k-steps of every node tile)
+    // of [Dr|Ds|Dt] for the first column tile
+    constexpr int KPRE = KC ? (S::RTV <= 3 ? 2 : 1) : 0;
+    R kfa[KC ? KPRE * 3 * S::RTV : 1];
     {
         using V4 = typename Mfma<R>::v4;
-        const R *lfr = a.LT + lane;
+        // node tiles outermost: each tile's fragments load once for all of
+        // the wave's elements, tile mt+1's while mt computes
+        R lfb[KSL];
 #pragma unroll
-        for (int c = 0; c < EPW; ++c) {
-            const int el = (NW - 1 - w) + c * NW;
-            if (el >= EB) break;
-            // LIFT fragments of node tile mt+1 load while mt computes
-            R la[KSL], lb[KSL];
+        for (int mt = 0; mt < MTL; ++mt) {
+            R *lc = (mt & 1) ? lfb : lfa;
+            R *ln = (mt & 1) ? lfa : lfb;
+            if (mt + 1 < MTL) {
 #pragma unroll
-            for (int ks = 0; ks < KSL; ++ks) la[ks] = lfr[ks * 64];
-#pragma unroll 1
-            for (int mt0 = 0; mt0 < MTL; mt0 += 2) {
+                for (int ks = 0; ks < KSL; ++ks) ln[ks] = lfr[((mt + 1) * KSL + ks) * 64];
+            } else if constexpr (!KC) {
+                // the last tile computes while the volume term's first
+                // operator fragments load
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int mt = mt0 + h;
-                    if (mt >= MTL) break;
-                    R *lc = h == 0 ? la : lb;
-                    R *ln = h == 0 ? lb : la;
-                    const int mn = mt + 1 < MTL ? mt + 1 : mt;
+                for (int ks = 0; ks < KS; ++ks) vfa[ks] = a.D[lane + ks * 64];
+            } else {
 #pragma unroll
-                    for (int ks = 0; ks < KSL; ++ks) ln[ks] = lfr[(mn * KSL + ks) * 64];
-                    V4 acc = {0, 0, 0, 0};
+                for (int p = 0; p < KPRE; ++p)
 #pragma unroll
-                    for (int ks = 0; ks < KSL; ++ks) acc = Mfma<R>::fma(lc[ks], fb[c][ks], acc);
-                    // accumulator element v: node row 4*(l/16)+v (FP32) or
-                    // 4v + l/16 (FP64) of the tile, face value l%16
+                    for (int h = 0; h < 3; ++h)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int node = mt * 16 + (sizeof(R) == 8 ? 4 * v + (lane >> 4) : 4 * (lane >> 4) + v);
-                        if (node < NP) sL[(lane & 15) * SLN + el * NP + node] = acc[v];
-                    }
+                        for (int rt = 0; rt < S::RTV; ++rt)
+                            kfa[(p * 3 + h) * S::RTV + rt] =
+                                a.DK[lane + (rt * S::KSV + (p < S::KSV / 3 ? p : 0) * 3 + h) * 64];
+            }
+#pragma unroll
+            for (int c = 0; c < EPW; ++c) {
+                const int el = (NW - 1 - w) + c * NW;
+                if (el >= EB) break;
+                V4 acc = {0, 0, 0, 0};
+#pragma unroll
+                for (int ks = 0; ks < KSL; ++ks) acc = Mfma<R>::fma(lc[ks], fb[c][ks], acc);
+                // accumulator element v: node row 4*(l/16)+v (FP32) or
+                // 4v + l/16 (FP64) of the tile, face value l%16
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int node = mt * 16 + (sizeof(R) == 8 ? 4 * v + (lane >> 4) : 4 * (lane >> 4) + v);
+                    if (node < NP) sL[(lane & 15) * SLN + el * NP + node] = acc[v];
                 }
             }
         }
@@ -795,6 +815,7 @@ stage_kernel(StageArgs<R> a) {
             // operator fragments of every node tile in chunks of CH k-steps,
             // the next chunk loading while this one computes
             constexpr int NB = PREF;   // chunks in flight
+            static_assert(NB - 1 == KPRE && CH == 3, "leading chunks are loaded during the LIFT");
             R abuf[NB][CH][RTV];
 #pragma unroll
             for (int p = 0; p < NB - 1; ++p)
@@ -802,7 +823,8 @@ stage_kernel(StageArgs<R> a) {
                 for (int h = 0; h < CH; ++h)
 #pragma unroll
                     for (int rt = 0; rt < RTV; ++rt)
-                        abuf[p][h][rt] = afr[(rt * KSV + (p < NCH ? p : 0) * CH + h) * 64];
+                        abuf[p][h][rt] = c == 0 ? kfa[(p * CH + h) * RTV + rt]
+                                                : afr[(rt * KSV + (p < NCH ? p : 0) * CH + h) * 64];
 #pragma unroll
             for (int ch = 0; ch < NCH; ++ch) {
                 if (ch + NB - 1 < NCH) {
@@ -852,9 +874,7 @@ stage_kernel(StageArgs<R> a) {
         const int lg = lane >> 4;
         // operator fragments double-buffered: row tile rt+1 loads while rt
         // computes (vector loads return in order)
-        R fa[KS], fb2[KS];
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks) fa[ks] = dfr[ks * 64];
+        R *fa = vfa, fb2[KS];
         R part[CPW][3];
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt) {
