@@ -180,6 +180,11 @@ __device__ __forceinline__ void upwind_flux(const R *fr, R vp, R vs, R rho, R la
 // RK2 stage 2 as well (1/2 base + 1/2 (a + dt*rhs), equal up to rounding),
 // which saves four operations per value; the FP32 memory variables and
 // fields use it.
+// fused multiply-add in the working precision (__builtin_fma is the double
+// one: on floats it would widen, fuse in FP64 and round back)
+__device__ __forceinline__ float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_r(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename R>
 struct RkStage {
     R wb, wx;    // wb * base + wx * (a + dt*rhs)
@@ -197,7 +202,7 @@ __device__ __forceinline__ R rk_combine(const RkStage<R> &s, R a, R base, R rhs,
 #pragma clang fp contract(off)
     const R k = rhs * dt;
     const R x = a + k;
-    const R z = __builtin_fma(s.wb, base, s.wx * x);
+    const R z = fma_r(s.wb, base, s.wx * x);
     if constexpr (WEIGHTED) return z;
     const R y = ((base + a) + k) * R(0.5);
     return s.avg ? y : z;
@@ -228,7 +233,7 @@ template <typename R>
 __device__ __forceinline__ void two_prod(R a, R b, R &p, R &e) {
 #pragma clang fp contract(off)
     p = a * b;
-    e = __builtin_fma(a, b, -p);
+    e = fma_r(a, b, -p);
 }
 template <typename R>
 __device__ __forceinline__ void rk_combine_comp(int scheme, int irk, R a, R alo, R base, R blo, R rhs, R dt,
