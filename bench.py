@@ -39,30 +39,26 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes_pml_att(np_, npf, w=8, rk_base=False):
-    """HBM bytes one PML+attenuation element needs in one RK stage: per node
-    read q 9, theta 18, f'/g'/h' 27, metric 9, PML coefficients 9 and write
-    q 9, theta 18, f'/g'/h' 27 (stages after the first also read the
-    step-start q, theta, f'/g'/h': 54 more); per element fscale 4*NpF, face
-    frames 36, material 5, anelastic 9 reals and 4*NpF+4 neighbour/face-type
-    ints.  Neighbour traces are other elements' q, read once already, so they
-    are not counted."""
-    per_node = (9 + 18 + 27 + 9 + 9) + (9 + 18 + 27) + (54 if rk_base else 0)
-    per_elem = 4 * npf + 36 + 5 + 9
-    return w * (np_ * per_node + per_elem) + 4 * (4 * npf + 4)
+def b_alg(np_, nv, w, r, nshot=1):
+    """Algorithmic HBM bytes of one element-stage for nshot shots of one
+    element: w*Np*nv*(2+r) per shot (read and write each wavefield, memory
+    variable and PML auxiliary once, plus the RK base on stages that read
+    it, r = 1), plus 54 reals of element metadata (9 affine metric terms, 4
+    fscale, 36 face frame components, 5 material values) and 48 bytes of
+    connectivity, shared by the shots of a launch (one geometry, one
+    material set)."""
+    return nshot * w * np_ * nv * (2 + r) + 54 * w + 48
 
 
-def algorithmic_bytes_att(np_, npf, w=8, rk_base=False):
-    """Same for an attenuating element outside the PML: q, theta, metric
-    read; q, theta written; the step-start q, theta read after stage 1."""
-    per_node = (9 + 18 + 9) + (9 + 18) + (27 if rk_base else 0)
-    per_elem = 4 * npf + 36 + 5 + 9
-    return w * (np_ * per_node + per_elem) + 4 * (4 * npf + 4)
+def b_alg_stage_mean(np_, nv, w, nrk=2, nshot=1):
+    """b_alg averaged over the nrk stages of a step: stage 1 reads no RK
+    base, the later ones do."""
+    return (b_alg(np_, nv, w, 0, nshot) + (nrk - 1) * b_alg(np_, nv, w, 1, nshot)) / nrk
 
 
-def stage_mean_bytes(fn, np_, npf, w, nrk=2):
-    """Algorithmic bytes per element averaged over the nrk stages of a step."""
-    return (fn(np_, npf, w, False) + (nrk - 1) * fn(np_, npf, w, True)) / nrk
+# wavefield values per node of a PML element with attenuation: 9 fields,
+# 18 memory variables, 27 PML auxiliaries
+NV_PML_ATT = 9 + 18 + 27
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_layered_box_n4.json")
@@ -316,6 +312,10 @@ def main():
     dof_step = p.nelem * p.ref.np_ * 9
     nsteps_all = args.warmup + args.steps
 
+    np_ = p.ref.np_
+    bpe64 = b_alg_stage_mean(np_, NV_PML_ATT, 8)                    # one shot per launch
+    bpe32 = b_alg_stage_mean(np_, NV_PML_ATT, 4, nshot=sb.nshot) / sb.nshot   # per element-shot
+
     # ---- FP64 plain path (the drop-in), shot 0 ------------------------------
     fp64 = None
     if not args.skip_fp64:
@@ -331,30 +331,39 @@ def main():
         ms_pml, n_pml = ctx.time_stage_kernel(1, nrep=10)
         ms_plain, n_plain = ctx.time_stage_kernel(0, nrep=10)
         ctx.close()
-        b_pml = stage_mean_bytes(algorithmic_bytes_pml_att, p.ref.np_, p.ref.npf, 8) * n_pml
         fp64 = dict(value=dof_step * args.steps / wall64, ms_per_step=wall64 / args.steps * 1e3,
                     ms_pml=ms_pml, n_pml=n_pml, ms_plain=ms_plain, n_plain=n_plain,
-                    achieved=b_pml / (ms_pml * 1e-3) / 1e9)
+                    achieved=bpe64 * n_pml / (ms_pml * 1e-3) / 1e9)
         log(f"[bench] FP64 plain: {fp64['ms_per_step']:.2f} ms/step, PML stage kernel {ms_pml:.3f} ms "
             f"({fp64['achieved']:.0f} GB/s), interior {ms_plain:.3f} ms")
 
-    # ---- FP32 batched path, all shots ----------------------------------------
-    bctx = capi.BatchContext(sb, precision=capi.NEXD_FP32, compat=capi.COMPAT_ALL)
-    bctx.reset()
-    if args.warmup:
-        bctx.run(1, args.warmup)
-    t1 = time.perf_counter()
-    bctx.run(args.warmup + 1, nsteps_all)
-    wall32 = time.perf_counter() - t1
-    seis32 = bctx.seismograms(0)[:, :, :nsteps_all]
-    ms32_pml, n32_pml = bctx.time_stage_kernel(1, nrep=5)
-    bctx.close()
-    value32 = dof_step * sb.nshot * args.steps / wall32
-    b32 = stage_mean_bytes(algorithmic_bytes_pml_att, p.ref.np_, p.ref.npf, 4) * n32_pml * sb.nshot
-    achieved32 = b32 / (ms32_pml * 1e-3) / 1e9
-    err32 = rel_l2_max(seis32, seis64) if fp64 else None
-    log(f"[bench] FP32 batched ({sb.nshot} shots): {wall32 / args.steps * 1e3:.2f} ms/step, PML stage kernel "
-        f"{ms32_pml:.3f} ms ({achieved32:.0f} GB/s), seismogram error vs FP64 over {nsteps_all} steps: {err32}")
+    # ---- FP32 batched path, all shots: the compensated RK stage sum and the
+    # energy of every step (as the reference computes it for PML runs) -------
+    def fp32_run(compensated, energy):
+        b = capi.BatchContext(sb, precision=capi.NEXD_FP32, compat=capi.COMPAT_ALL, compensated=compensated,
+                              energy=energy)
+        b.reset()
+        if args.warmup:
+            b.run(1, args.warmup)
+        t = time.perf_counter()
+        b.run(args.warmup + 1, nsteps_all)
+        wall = time.perf_counter() - t
+        seis = b.seismograms(0)[:, :, :nsteps_all]
+        ms_k, n_k = b.time_stage_kernel(1, nrep=5)
+        b.close()
+        return dict(wall=wall, seis=seis, ms_pml=ms_k, n_pml=n_k, value=dof_step * sb.nshot * args.steps / wall,
+                    ms_per_step=wall / args.steps * 1e3,
+                    achieved=bpe32 * n_k * sb.nshot / (ms_k * 1e-3) / 1e9)
+
+    f32 = fp32_run(True, True)
+    value32, wall32, ms32_pml, n32_pml, achieved32 = (f32["value"], f32["wall"], f32["ms_pml"], f32["n_pml"],
+                                                       f32["achieved"])
+    err32 = rel_l2_max(f32["seis"], seis64) if fp64 else None
+    log(f"[bench] FP32 batched ({sb.nshot} shots, compensated, energy): {f32['ms_per_step']:.2f} ms/step, PML stage "
+        f"kernel {ms32_pml:.3f} ms ({achieved32:.0f} GB/s), seismogram error vs FP64 over {nsteps_all} steps: {err32}")
+    f32u = fp32_run(False, False)
+    log(f"[bench] FP32 batched ({sb.nshot} shots, plain sum, no energy): {f32u['ms_per_step']:.2f} ms/step, PML "
+        f"stage kernel {f32u['ms_pml']:.3f} ms ({f32u['achieved']:.0f} GB/s)")
 
     accuracy = None
     if args.accuracy_nt:
@@ -363,7 +372,7 @@ def main():
     # HBM bytes per launch measured by rocprofv3 counters on this workload
     # (committed summary, made with --pmc-summary); null when absent
     traffic = traffic64 = None
-    workload_key = f"{args.nx}x{args.ny}x{args.nz}/{sb.nshot}"
+    workload_key = f"{args.nx}x{args.ny}x{args.nz}/{sb.nshot}/compensated"
     if os.path.exists(TRAFFIC_FILE):
         tf = json.load(open(TRAFFIC_FILE))
         if tf.get("workload") == workload_key:
@@ -372,8 +381,6 @@ def main():
                     traffic = v["bytes_per_launch"]
                 elif "<4, true, true, double>" in name:
                     traffic64 = v["bytes_per_launch"]
-    bpe32 = stage_mean_bytes(algorithmic_bytes_pml_att, p.ref.np_, p.ref.npf, 4)
-    bpe64 = stage_mean_bytes(algorithmic_bytes_pml_att, p.ref.np_, p.ref.npf, 8)
     out = {
         "metric": "DOF-updates/s on 1 GPU: FP64 plain vs FP32 batched at ≤1e-4 seismogram L2",
         "value": value32,
@@ -389,19 +396,27 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"two-layer anelastic NPML box {args.nx}x{args.ny}x{args.nz} hexes "
                                f"({p.nelem} tets), N=4, 3 SLS, RK2; FP32 batched path with {sb.nshot} shots "
-                               f"(one moment-tensor source each) in one launch per stage",
+                               f"(one moment-tensor source each) in one launch per stage, compensated RK stage "
+                               f"sum, energy of every step",
                    "dof_per_step": dof_step * sb.nshot, "rk_stages": 2, "shots": sb.nshot,
                    "fp64_plain_dof_updates_per_s": fp64["value"] if fp64 else None,
                    "fp64_plain_ms_per_step": fp64["ms_per_step"] if fp64 else None,
                    "fp32_over_fp64_shot_throughput": (value32 / fp64["value"]) if fp64 else None,
                    "fp32_seismogram_rel_l2_vs_fp64": err32,
+                   "fp32_plain_sum_no_energy": {"dof_updates_per_s": f32u["value"], "ms_per_step": f32u["ms_per_step"],
+                                                "pml_kernel_ms": f32u["ms_pml"],
+                                                "roofline_frac": f32u["achieved"] / HBM_PEAK_GBS},
                    "accuracy_steps": nsteps_all,
                    "accuracy": accuracy},
         "roofline": {"bound": "hbm", "achieved": achieved32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved32 / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "stage_kernel<4,ATT,PML,f32> (PML elements, all shots; mean over the 2 RK stages)",
-                     "bytes_per_element": bpe32, "algorithmic_bytes_per_launch": bpe32 * n32_pml * sb.nshot,
-                     "avg_launch_ms": ms32_pml, "elements_per_launch": n32_pml * sb.nshot,
+                     "kernel": "stage_kernel<4,ATT,PML,f32> (PML elements, all shots, compensated; mean over the 2 "
+                               "RK stages, energy partials on the second)",
+                     "bytes_per_element_shot": bpe32,
+                     "bytes_model": "w*Np*54*(2+r) per element-shot (r: RK base read on stage 2) + (54*w + 48) per "
+                                    "element shared by the shots",
+                     "algorithmic_bytes_per_launch": bpe32 * n32_pml * sb.nshot,
+                     "avg_launch_ms": ms32_pml, "elements_per_launch": n32_pml, "shots_per_launch": sb.nshot,
                      "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if traffic else None,
                      "fp64_kernel": ({"achieved": fp64["achieved"], "frac": fp64["achieved"] / HBM_PEAK_GBS,
                                       "avg_launch_ms": fp64["ms_pml"], "elements_per_launch": fp64["n_pml"],

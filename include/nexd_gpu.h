@@ -34,10 +34,13 @@ extern "C" {
 #define NEXD_E_UNSUPPORTED (-4)/* feature outside what the library implements */
 
 /* timeint values: the reference's 1 (Euler), 2 (TVD RK2), 3 (TVD RK3)
- * (src/timeloopMod.f90:230-248) */
+ * (src/timeloopMod.f90:230-248), and 4, an extension without a reference
+ * counterpart: the five-stage fourth-order low-storage (2N-storage) RK of
+ * Carpenter and Kennedy (NASA TM-109112, 1994) */
 #define NEXD_EULER 1
 #define NEXD_RK2 2
 #define NEXD_RK3 3
+#define NEXD_LSRK4 4
 
 /* compat bits: reproduce reference behaviour bit for bit */
 #define NEXD_COMPAT_RK3_FLOAT_WEIGHTS 1  /* 1/3, 2/3 rounded to float (src/timeloopMod.f90:129-133) */
@@ -51,7 +54,7 @@ typedef struct nexd_config {
     int nsrc;              /* point sources in this partition */
     int nrec;              /* receivers in this partition */
     int nt;                /* time steps (par%nt) */
-    int timeint;           /* NEXD_EULER / NEXD_RK2 / NEXD_RK3 (par%timeint) */
+    int timeint;           /* NEXD_EULER / NEXD_RK2 / NEXD_RK3 (par%timeint) / NEXD_LSRK4 */
     int attenuation;       /* par%attenuation: 3 standard linear solids */
     int pml;               /* par%set_pml */
     int compat;            /* NEXD_COMPAT_* bits */
@@ -192,6 +195,8 @@ typedef struct nexd_batch_config {
     int shared_material;  /* 1: one material/PML set for every shot */
     int compensated;      /* 1: compensated RK stage sum (state kept as hi+lo
                              pairs of the working precision) */
+    int energy;           /* with pml: each shot's energy after every step
+                             (src/timeloopMod.f90:508-516), nexd_batch_get_energy */
     int device;
     double dt;
 } nexd_batch_config;
@@ -209,9 +214,11 @@ int nexd_batch_upload_mesh(nexd_batch *b, const int *ibool,
                            const double *jacobian, const double *fscale,
                            const int *neighbor, const int *face, const int *ibt, const int *ibn,
                            const double *nnormal, const double *stangential, const double *ttangential);
-/* per shot (shot 0..nshot-1; with shared_material only shot 0) */
+/* per shot (shot 0..nshot-1; with shared_material only shot 0); the same
+ * arrays as nexd_upload_material (mu, lambda enter only the energy) */
 int nexd_batch_upload_material(nexd_batch *b, int shot, const double *vpu, const double *vsu,
-                               const double *rho, const double *lambdau, const double *muu);
+                               const double *rho, const double *lambdau, const double *muu,
+                               const double *mu, const double *lambda);
 int nexd_batch_upload_anelastic(nexd_batch *b, int shot, const double *wl, const double *anelasticvar);
 /* pmlcheck is a mesh property and must be the same for every shot */
 int nexd_batch_upload_pml(nexd_batch *b, int shot, const int *pmlcheck,
@@ -228,6 +235,12 @@ int nexd_batch_run(nexd_batch *b, int it_begin, int it_end);
 /* seismograms of one shot, each (nrec,nt) in double */
 int nexd_batch_get_seismograms(nexd_batch *b, int shot, double *ux, double *uy, double *uz,
                                double *vx, double *vy, double *vz, double *ax, double *ay, double *az);
+/* energy of one shot after every step recorded so far (nt values; config
+ * energy and pml), as nexd_run returns it for one context */
+int nexd_batch_get_energy(nexd_batch *b, int shot, double *energy);
+/* one shot's PML switch for its STA/LTA trigger (src/timeloopMod.f90:532-539),
+ * as nexd_set_pml_active for one context; the other shots keep theirs */
+int nexd_batch_set_pml_active(nexd_batch *b, int shot, int active);
 int nexd_batch_last_run_ms(nexd_batch *b, double *ms);
 /* measurement: mean duration of one stage launch of element list 0/1,
  * averaged over the RK stages (as nexd_time_stage_kernel); overwrites the
@@ -254,6 +267,39 @@ int nexd_group_run(nexd_group *g, int it_begin, int it_end);
 int nexd_group_last_run_ms(nexd_group *g, double *ms);
 void nexd_group_destroy(nexd_group *g);
 const char *nexd_group_last_error(const nexd_group *g);
+
+/* ==========================================================================
+ * Multilayer perceptron (no reference counterpart): the learned
+ * source-to-seismogram surrogate, trained and evaluated by the library's own
+ * kernels.  widths[0..nlayers]: input, hidden and output widths; hidden
+ * layers apply GELU (erf form), the output layer is linear.  Weights are
+ * FP32 master copies with BF16 operands for the matrix-core products (FP32
+ * accumulation); Adam updates.  Arrays are host, C row-major: W [out][in],
+ * b [out], x [B][in], y and targets [B][out].  Batches B are multiples of 64
+ * up to max_batch; training needs an output width that is a multiple of 64.
+ * ========================================================================== */
+typedef struct nexd_mlp nexd_mlp;
+
+int nexd_mlp_create(int nlayers, const int *widths, int max_batch, int device, nexd_mlp **out);
+/* sets one layer's weights and biases (and restarts the optimiser) */
+int nexd_mlp_set_params(nexd_mlp *m, int layer, const float *W, const float *b);
+/* which 0: weights and biases; 1: their gradients from the last
+ * nexd_mlp_gradients or training step.  W or b may be NULL. */
+int nexd_mlp_get_params(nexd_mlp *m, int layer, int which, float *W, float *b);
+int nexd_mlp_forward(nexd_mlp *m, const float *x, int B, float *y);
+/* the training set, kept on the device: x [n][in], targets [n][out] */
+int nexd_mlp_upload_dataset(nexd_mlp *m, const float *x, const float *t, int n);
+/* mean-squared-error loss and all gradients of the dataset rows idx[0..B),
+ * without an update */
+int nexd_mlp_gradients(nexd_mlp *m, const int *idx, int B, float *loss);
+/* nsteps Adam steps over consecutive batches idx[s*B .. s*B+B); losses[s]
+ * (may be NULL) is the loss of step s before its update */
+int nexd_mlp_train(nexd_mlp *m, const int *idx, int B, int nsteps, float lr, float beta1, float beta2, float eps,
+                   float *losses);
+/* device time of the last nexd_mlp_train in milliseconds */
+int nexd_mlp_last_run_ms(nexd_mlp *m, double *ms);
+void nexd_mlp_destroy(nexd_mlp *m);
+const char *nexd_mlp_last_error(const nexd_mlp *m);
 
 #ifdef __cplusplus
 }

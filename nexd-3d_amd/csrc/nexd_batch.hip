@@ -15,6 +15,7 @@
 #include "../../include/nexd_gpu.h"
 #include "host_layout.hpp"
 #include "plain_kernels.hpp"
+#include "stage_launch.hpp"
 
 namespace {
 
@@ -44,6 +45,7 @@ struct Batch : BatchBase {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<void *> allocs;
     R *q[3] = {nullptr, nullptr, nullptr};
+    R *qres = nullptr, *thres = nullptr, *pres = nullptr;   // low-storage RK4 residuals
     R *qlo[3] = {nullptr, nullptr, nullptr};   // low parts (compensated state)
     R *u_slot_lo = nullptr;
     R *th[2] = {nullptr, nullptr};
@@ -52,7 +54,7 @@ struct Batch : BatchBase {
     R *jac = nullptr;
     int *fmask = nullptr, *nbr = nullptr, *ftype = nullptr;
     std::vector<int> fmask_host;
-    R *matc = nullptr, *anel = nullptr, *pmlc = nullptr;
+    R *matc = nullptr, *anel = nullptr, *pmlc = nullptr, *emat = nullptr;
     double wl[nexd::NMB] = {0, 0, 0};
     std::vector<int> pmlcheck;
     int *elist_plain = nullptr, *elist_pml = nullptr;
@@ -62,6 +64,10 @@ struct Batch : BatchBase {
     int *recelem = nullptr, *rec_slot = nullptr, *rec_slot_of = nullptr;
     R *rec_int = nullptr, *u_slot = nullptr, *a_slot = nullptr, *seis = nullptr, *last_flux = nullptr;
     int *d_it = nullptr;
+    int *pml_on = nullptr;                    // [shot] 1 while the shot's PML is on
+    std::vector<int> pml_on_host;
+    double *blk_energy = nullptr, *energy = nullptr;   // [shot][nblk_total], [shot][nt] (FP64 sums)
+    int nblk_total = 0;
     int step = 0;
     bool have_ops = false, have_mesh = false, initialized = false;
     std::vector<char> have_mat, have_att, have_pml, have_src;
@@ -127,7 +133,7 @@ struct Batch : BatchBase {
         npf = (c.order + 1) * (c.order + 2) / 2;
         nfp = 4 * npf;
         ng = (size_t)c.nelem * np;
-        nrk = c.timeint == 1 ? 1 : (c.timeint == 2 ? 2 : 3);
+        nrk = c.timeint == 1 ? 1 : (c.timeint == 2 ? 2 : (c.timeint == 3 ? 3 : 5));
         if (hipSetDevice(c.device) != hipSuccess) return fail(NEXD_E_HIP, "hipSetDevice failed");
         if (hipStreamCreate(&stream) != hipSuccess || hipStreamCreate(&side) != hipSuccess)
             return fail(NEXD_E_HIP, "hipStreamCreate failed");
@@ -140,11 +146,19 @@ struct Batch : BatchBase {
         for (int i = 0; i < 3 && !rc; ++i) rc = dalloc(&q[i], S * 9 * ng);
         for (int i = 0; i < 3 && !rc && c.compensated; ++i) rc = dalloc(&qlo[i], S * 9 * ng);
         if (!rc && c.attenuation) (rc = dalloc(&th[0], S * 18 * ng)) || (rc = dalloc(&th[1], S * 18 * ng));
+        if (!rc && c.timeint == NEXD_LSRK4) {
+            rc = dalloc(&qres, S * 9 * ng);
+            if (!rc && c.attenuation) rc = dalloc(&thres, S * 18 * ng);
+            if (!rc && c.pml) rc = dalloc(&pres, S * 27 * ng);
+        }
         if (!rc && c.pml)
             (rc = dalloc(&paux[0], S * 27 * ng)) || (rc = dalloc(&paux[1], S * 27 * ng)) ||
                 (rc = dalloc(&pmlc, (size_t)nmat_sets() * 9 * ng)) || (rc = dalloc(&last_flux, S * 27 * np));
+        if (!rc && c.pml)
+            (rc = dalloc(&pml_on, S)) || (rc = dalloc(&blk_energy, S * ((size_t)c.nelem + 64))) ||
+                (rc = dalloc(&energy, S * c.nt));
         if (!rc)
-            (rc = dalloc(&matc, (size_t)nmat_sets() * c.nelem * 8)) ||
+            (rc = dalloc(&matc, (size_t)nmat_sets() * c.nelem * 8)) || (rc = dalloc(&emat, (size_t)nmat_sets() * c.nelem * 2)) ||
                 (rc = dalloc(&anel, (size_t)nmat_sets() * c.nelem * nexd::NMB * 3)) ||
                 (rc = dalloc(&elist_plain, c.nelem)) || (rc = dalloc(&elist_pml, c.nelem)) || (rc = dalloc(&d_it, 1)) ||
                 (rc = dalloc(&seis, S * 9 * c.nt * (c.nrec > 0 ? c.nrec : 1))) ||
@@ -162,7 +176,14 @@ struct Batch : BatchBase {
         have_rec = c.nrec == 0;
         std::vector<int> minus(S * c.nelem, -1);
         if ((rc = put(src_head, minus)) || (rc = put(rec_slot, std::vector<int>(c.nelem, -1)))) return rc;
+        pml_on_host.assign(S, 1);
+        if (c.pml && (rc = put(pml_on, pml_on_host))) return rc;
         return rebuild_lists();
+    }
+
+    // element blocks of one list launch (per shot)
+    int nblocks(int n, bool pml) const {
+        return nexd::stage_blocks<R>(cfg.order, n, cfg.attenuation != 0, pml);
     }
 
     int rebuild_lists() {
@@ -174,9 +195,14 @@ struct Batch : BatchBase {
         int rc;
         if (n_plain && (rc = put(elist_plain, plain))) return rc;
         if (n_pml && (rc = put(elist_pml, pmll))) return rc;
+        nblk_total = nblocks(n_plain, false) + nblocks(n_pml, true);
         drop_graph();
         return 0;
     }
+
+    // the reference computes the energy after every step of a run with PML
+    // (src/timeloopMod.f90:482-516)
+    bool energy_on() const { return cfg.energy && cfg.pml; }
 
     int check_shot(int shot) {
         if (cfg.shared_material ? shot != 0 : (shot < 0 || shot >= cfg.nshot))
@@ -211,36 +237,18 @@ struct Batch : BatchBase {
         a.d_it = d_it;
         a.rec_slot = rec_slot; a.u_slot = u_slot; a.a_slot = a_slot; a.u_full = nullptr;
         a.blk_energy = nullptr;
-        a.mass = mass; a.jac = jac; a.emat = nullptr;
+        a.blk_stride = nblk_total;
+        a.mass = mass; a.jac = jac; a.emat = emat;
+        a.emat_stride = cfg.shared_material ? 0 : cfg.nelem * 2;
+        a.pml_on = cfg.pml ? pml_on : nullptr;
         return a;
-    }
-
-    template <int ORDER, bool ATT, bool PML>
-    void launch_one(const nexd::StageArgs<R> &a, hipStream_t s) {
-        using Sh = nexd::Shape<ORDER, ATT, PML, R>;
-        const unsigned grid = (unsigned)((a.nlist + Sh::EB - 1) / Sh::EB) * (unsigned)a.nshot;
-        if (grid == 0) return;
-        hipLaunchKernelGGL((nexd::stage_kernel<ORDER, ATT, PML, R>), dim3(grid), dim3(Sh::NTHR), 0, s, a);
-    }
-
-    template <int ORDER>
-    void stage(const nexd::StageArgs<R> &a, bool pml, hipStream_t s) {
-        const bool att = cfg.attenuation != 0;
-        if (att && pml) launch_one<ORDER, true, true>(a, s);
-        else if (att) launch_one<ORDER, true, false>(a, s);
-        else if (pml) launch_one<ORDER, false, true>(a, s);
-        else launch_one<ORDER, false, false>(a, s);
     }
 
     void dispatch(const nexd::StageArgs<R> &a, bool pml) { dispatch(a, pml, stream); }
     void dispatch(const nexd::StageArgs<R> &a, bool pml, hipStream_t s) {
-        switch (cfg.order) {
-            case 1: stage<1>(a, pml, s); break;
-            case 2: stage<2>(a, pml, s); break;
-            case 3: stage<3>(a, pml, s); break;
-            case 4: stage<4>(a, pml, s); break;
-            default: stage<5>(a, pml, s); break;
-        }
+        const bool att = cfg.attenuation != 0;
+        if (a.scheme == NEXD_LSRK4) nexd::launch_stage<R, true>(cfg.order, att, pml, a, s);
+        else nexd::launch_stage<R, false>(cfg.order, att, pml, a, s);
     }
 
     template <int ORDER>
@@ -265,7 +273,8 @@ struct Batch : BatchBase {
     void step_end_impl(int row_offset, int advance) {
         hipLaunchKernelGGL((nexd::step_end_kernel<ORDER, R>), dim3(1), dim3(1024), 0, stream, q[0], u_slot, a_slot,
                            rec_int, recelem, rec_slot_of, cfg.nrec, nslot, cfg.nshot, (int)ng, cfg.nt, row_offset,
-                           advance, seis, (const R *)nullptr, 0, (R *)nullptr, d_it);
+                           advance, seis, energy_on() ? (const double *)blk_energy : (const double *)nullptr, nblk_total, energy,
+                           d_it);
     }
 
     void step_end(int row_offset, int advance) {
@@ -286,16 +295,26 @@ struct Batch : BatchBase {
         } else if (scheme == NEXD_RK2) {
             if (irk == 1) { qin = q[0]; qout = q[1]; thin = th[0]; thout = th[1]; pin = paux[0]; pout = paux[1]; }
             else { qin = q[1]; qout = q[0]; thin = th[1]; thout = th[0]; pin = paux[1]; pout = paux[0]; }
-        } else {
+        } else if (scheme == NEXD_RK3) {
             if (irk == 1) { qin = q[0]; qout = q[1]; thin = th[0]; thout = th[1]; pin = paux[0]; pout = paux[1]; }
             else if (irk == 2) { qin = q[1]; qout = q[2]; thin = th[1]; thout = th[1]; pin = paux[1]; pout = paux[1]; }
             else { qin = q[2]; qout = q[0]; thin = th[1]; thout = th[0]; pin = paux[1]; pout = paux[0]; }
+        } else {
+            // low-storage RK4: q rotates 0 -> 1 -> 2 -> 1 -> 2 -> 0; memory
+            // variables and PML auxiliaries update in place
+            static const int from[5] = {0, 1, 2, 1, 2}, to[5] = {1, 2, 1, 2, 0};
+            qin = q[from[irk - 1]]; qout = q[to[irk - 1]];
+            thin = thout = th[0]; pin = pout = paux[0];
         }
         a.irk = irk;
         a.last_stage = irk == nrk ? 1 : 0;
-        a.qin = qin; a.qbase = q[0]; a.qout = qout;
-        a.thin = thin; a.thbase = th[0]; a.thout = thout;
-        a.pin = pin; a.pbase = paux[0]; a.pout = pout;
+        const bool ls = scheme == NEXD_LSRK4;
+        a.qin = qin; a.qbase = ls ? qres : q[0]; a.qout = qout;
+        a.thin = thin; a.thbase = ls ? thres : th[0]; a.thout = thout;
+        a.pin = pin; a.pbase = ls ? pres : paux[0]; a.pout = pout;
+        a.qres = ls ? qres : nullptr; a.thres = ls ? thres : nullptr; a.pres = ls ? pres : nullptr;
+        a.lsa = ls ? (R)nexd::kLsrkA[irk - 1] : R(0);
+        a.lsb = ls ? (R)nexd::kLsrkB[irk - 1] : R(0);
         if (cfg.compensated) {
             auto lo = [&](const R *p) { return p == q[0] ? qlo[0] : (p == q[1] ? qlo[1] : qlo[2]); };
             a.qlo_in = lo(qin); a.qlo_base = qlo[0]; a.qlo_out = lo(qout);
@@ -311,18 +330,22 @@ struct Batch : BatchBase {
             // the interior list runs on a side stream, beside the PML list:
             // both read the stage input and write disjoint elements
             const bool fork = pml && n_plain > 0;
+            // energy partials of the last stage: [shot][plain blocks, PML blocks]
+            const bool en = a.last_stage && energy_on();
             if (fork) {
                 (void)hipEventRecord(ev_fork, stream);
                 (void)hipStreamWaitEvent(side, ev_fork, 0);
             }
             a.elist = elist_plain;
             a.nlist = n_plain;
+            a.blk_energy = en ? blk_energy : nullptr;
             dispatch(a, false, fork ? side : stream);
             if (fork) (void)hipEventRecord(ev_join, side);
             if (pml) {
                 if (a.pml_compat) last_flux_dispatch(a.qin, a.pin);
                 a.elist = elist_pml;
                 a.nlist = n_pml;
+                a.blk_energy = en ? blk_energy + nblocks(n_plain, false) : nullptr;
                 dispatch(a, true);
             }
             if (fork) (void)hipStreamWaitEvent(stream, ev_join, 0);
@@ -362,6 +385,12 @@ struct Batch : BatchBase {
                 int rc = put(paux[i], std::vector<R>(S * 27 * ng, R(0)));
                 if (rc) return rc;
             }
+        if (qres) {
+            int rc;
+            if ((rc = put(qres, std::vector<R>(S * 9 * ng, R(0))))) return rc;
+            if (thres && (rc = put(thres, std::vector<R>(S * 18 * ng, R(0))))) return rc;
+            if (pres && (rc = put(pres, std::vector<R>(S * 27 * ng, R(0))))) return rc;
+        }
         if (nslot) {
             std::vector<R> t(S * nslot * 3 * np, tiny);
             int rc;
@@ -370,6 +399,7 @@ struct Batch : BatchBase {
         }
         int rc = put(seis, std::vector<R>(S * 9 * cfg.nt * (cfg.nrec > 0 ? cfg.nrec : 1), R(0)));
         if (rc) return rc;
+        if (energy && (rc = put(energy, std::vector<double>(S * cfg.nt, 0.0)))) return rc;
         if ((rc = put(d_it, std::vector<int>(1, 0)))) return rc;
         step = 0;
         initialized = true;
@@ -441,6 +471,7 @@ struct Batch : BatchBase {
         const bool pml = cfg.pml && n_pml > 0;
         if (list == 1 && !pml) return fail(NEXD_E_STATE, "no PML element list");
         nexd::StageArgs<R> a = base_args();
+        a.blk_energy = nullptr;
         a.elist = list == 0 ? elist_plain : elist_pml;
         a.nlist = list == 0 ? n_plain : n_pml;
         // every RK stage in turn, as the time loop runs them, after a stage-1
@@ -449,9 +480,12 @@ struct Batch : BatchBase {
         if (list == 1 && a.pml_compat) last_flux_dispatch(a.qin, a.pin);
         dispatch(a, list == 1);
         if (hipEventRecord(ev0, stream) != hipSuccess) return fail(NEXD_E_HIP, "hipEventRecord failed");
+        // the last stage writes its energy partials, as in the time loop
+        double *en = energy_on() ? blk_energy + (list == 1 ? nblocks(n_plain, false) : 0) : nullptr;
         for (int r = 0; r < nrep; ++r)
             for (int irk = 1; irk <= nrk; ++irk) {
                 set_stage_buffers(a, irk);
+                a.blk_energy = a.last_stage ? en : nullptr;
                 dispatch(a, list == 1);
             }
         if (hipEventRecord(ev1, stream) != hipSuccess) return fail(NEXD_E_HIP, "hipEventRecord failed");
@@ -515,18 +549,20 @@ int upload_mesh(Batch<R> *b, const int *ibool, const double *const met[9], const
 
 template <typename R>
 int upload_material(Batch<R> *b, int shot, const double *vpu, const double *vsu, const double *rho,
-                    const double *lambdau, const double *muu) {
+                    const double *lambdau, const double *muu, const double *mu, const double *lambda) {
     int rc = b->check_shot(shot);
     if (rc) return rc;
-    if (!vpu || !vsu || !rho || !lambdau || !muu) return b->fail(NEXD_E_ARG, "null material");
+    if (!vpu || !vsu || !rho || !lambdau || !muu || !mu || !lambda) return b->fail(NEXD_E_ARG, "null material");
     const int nel = b->cfg.nelem;
-    std::vector<R> mc((size_t)nel * 8, R(0));
+    std::vector<R> mc((size_t)nel * 8, R(0)), em((size_t)nel * 2);
     for (int e = 0; e < nel; ++e) {
         if (!(rho[e] > 0)) return b->fail(NEXD_E_ARG, "density must be positive");
         mc[e * 8 + 0] = (R)vpu[e]; mc[e * 8 + 1] = (R)vsu[e]; mc[e * 8 + 2] = (R)rho[e];
         mc[e * 8 + 3] = (R)lambdau[e]; mc[e * 8 + 4] = (R)muu[e];
+        em[e * 2] = (R)mu[e]; em[e * 2 + 1] = (R)lambda[e];
     }
-    if ((rc = b->put(b->matc + (size_t)shot * nel * 8, mc))) return rc;
+    if ((rc = b->put(b->matc + (size_t)shot * nel * 8, mc)) || (rc = b->put(b->emat + (size_t)shot * nel * 2, em)))
+        return rc;
     b->have_mat[shot] = 1;
     b->drop_graph();
     return 0;
@@ -560,15 +596,18 @@ int upload_pml(Batch<R> *b, int shot, const int *pmlcheck, const double *const p
     if (!pmlcheck) return b->fail(NEXD_E_ARG, "null PML array");
     for (int k = 0; k < 9; ++k)
         if (!prof[k]) return b->fail(NEXD_E_ARG, "null PML array");
+    // the mask is a mesh property: checked against the shots uploaded so
+    // far before it replaces the stored one
+    std::vector<int> mask(b->cfg.nelem);
     bool changed = false;
     for (int e = 0; e < b->cfg.nelem; ++e) {
-        const int v = pmlcheck[e] ? 1 : 0;
-        if (v != b->pmlcheck[e]) changed = true;
-        b->pmlcheck[e] = v;
+        mask[e] = pmlcheck[e] ? 1 : 0;
+        if (mask[e] != b->pmlcheck[e]) changed = true;
     }
     bool any_before = false;
     for (char c : b->have_pml) any_before |= c != 0;
     if (changed && any_before) return b->fail(NEXD_E_ARG, "pmlcheck must be the same for every shot");
+    b->pmlcheck.swap(mask);
     std::vector<double> pc;
     nexd::pml_coefficients(b->ng, prof[0], prof[1], prof[2], prof[3], prof[4], prof[5], prof[6], prof[7], prof[8], pc);
     if ((rc = b->put(b->pmlc + (size_t)shot * 9 * b->ng, cvt<R>(pc)))) return rc;
@@ -648,6 +687,25 @@ int get_seis(Batch<R> *b, int shot, double *const outs[9]) {
         if (e != hipSuccess) return b->fail(NEXD_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
         for (size_t i = 0; i < n; ++i) outs[k][i] = (double)h[i];
     }
+    return 0;
+}
+
+template <typename R>
+int get_energy(Batch<R> *b, int shot, double *out) {
+    if (shot < 0 || shot >= b->cfg.nshot || !out) return b->fail(NEXD_E_ARG, "shot out of range or null output");
+    if (!b->energy_on()) return b->fail(NEXD_E_STATE, "batch created without energy or without PML");
+    hipError_t e = hipMemcpy(out, b->energy + (size_t)shot * b->cfg.nt, sizeof(double) * b->cfg.nt, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return b->fail(NEXD_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    return 0;
+}
+
+template <typename R>
+int set_pml_active(Batch<R> *b, int shot, int active) {
+    if (shot < 0 || shot >= b->cfg.nshot) return b->fail(NEXD_E_ARG, "shot out of range");
+    if (!b->cfg.pml) return b->fail(NEXD_E_STATE, "batch created without PML");
+    b->pml_on_host[shot] = active ? 1 : 0;
+    hipError_t e = hipMemcpy(b->pml_on + shot, &b->pml_on_host[shot], sizeof(int), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return b->fail(NEXD_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
     return 0;
 }
 
@@ -764,7 +822,14 @@ int nexd_batch_create(const nexd_batch_config *cfg, nexd_batch **out) {
         g_batch_create_error = "nelem, nt, nshot must be positive; nsrc, nrec non-negative";
         return NEXD_E_ARG;
     }
-    if (cfg->timeint < 1 || cfg->timeint > 3) { g_batch_create_error = "timeint must be 1, 2 or 3"; return NEXD_E_UNSUPPORTED; }
+    if (cfg->timeint < 1 || cfg->timeint > 4) {
+        g_batch_create_error = "timeint must be 1, 2, 3 or 4 (low-storage RK4)";
+        return NEXD_E_UNSUPPORTED;
+    }
+    if (cfg->timeint == NEXD_LSRK4 && cfg->compensated) {
+        g_batch_create_error = "the compensated stage sum is built for the TVD schemes only";
+        return NEXD_E_UNSUPPORTED;
+    }
     if (cfg->precision != NEXD_FP32 && cfg->precision != NEXD_FP64) {
         g_batch_create_error = "precision must be NEXD_FP32 or NEXD_FP64";
         return NEXD_E_ARG;
@@ -815,10 +880,11 @@ int nexd_batch_upload_mesh(nexd_batch *b, const int *ibool, const double *rx, co
 }
 
 int nexd_batch_upload_material(nexd_batch *b, int shot, const double *vpu, const double *vsu, const double *rho,
-                               const double *lambdau, const double *muu) {
+                               const double *lambdau, const double *muu, const double *mu, const double *lambda) {
     if (!b) return NEXD_E_ARG;
-    return base(b)->cfg.precision == NEXD_FP32 ? upload_material(as<float>(b), shot, vpu, vsu, rho, lambdau, muu)
-                                               : upload_material(as<double>(b), shot, vpu, vsu, rho, lambdau, muu);
+    return base(b)->cfg.precision == NEXD_FP32
+               ? upload_material(as<float>(b), shot, vpu, vsu, rho, lambdau, muu, mu, lambda)
+               : upload_material(as<double>(b), shot, vpu, vsu, rho, lambdau, muu, mu, lambda);
 }
 
 int nexd_batch_upload_anelastic(nexd_batch *b, int shot, const double *wl, const double *anelasticvar) {
@@ -864,6 +930,18 @@ int nexd_batch_get_seismograms(nexd_batch *b, int shot, double *ux, double *uy, 
     if (!b) return NEXD_E_ARG;
     double *const outs[9] = {ux, uy, uz, vx, vy, vz, ax, ay, az};
     return base(b)->cfg.precision == NEXD_FP32 ? get_seis(as<float>(b), shot, outs) : get_seis(as<double>(b), shot, outs);
+}
+
+int nexd_batch_get_energy(nexd_batch *b, int shot, double *energy) {
+    if (!b) return NEXD_E_ARG;
+    return base(b)->cfg.precision == NEXD_FP32 ? get_energy(as<float>(b), shot, energy)
+                                               : get_energy(as<double>(b), shot, energy);
+}
+
+int nexd_batch_set_pml_active(nexd_batch *b, int shot, int active) {
+    if (!b) return NEXD_E_ARG;
+    return base(b)->cfg.precision == NEXD_FP32 ? set_pml_active(as<float>(b), shot, active)
+                                               : set_pml_active(as<double>(b), shot, active);
 }
 
 int nexd_batch_last_run_ms(nexd_batch *b, double *ms) {

@@ -10,11 +10,13 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/nexd_gpu.h"
 #include "host_layout.hpp"
 #include "plain_kernels.hpp"
+#include "stage_launch.hpp"
 
 using nexd::Dims;
 using nexd::StageArgs;
@@ -37,9 +39,10 @@ struct nexd_ctx {
     size_t nglob = 0;
     hipStream_t stream = nullptr;
     std::string err;
-    std::vector<void *> allocs;
+    std::vector<std::pair<void *, size_t>> allocs;   // device buffers and their sizes in bytes
     // wavefield
     double *q[3] = {nullptr, nullptr, nullptr};
+    double *qres = nullptr, *thres = nullptr, *pres = nullptr;   // low-storage RK4 residuals
     double *th[2] = {nullptr, nullptr};
     double *paux[2] = {nullptr, nullptr};
     double *u_full = nullptr;
@@ -97,17 +100,35 @@ int dalloc(nexd_ctx *c, T **p, size_t n) {
     if (n == 0) n = 1;
     hipError_t e = hipMalloc(&v, n * sizeof(T));
     if (e != hipSuccess) return fail(c, NEXD_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
-    c->allocs.push_back(v);
+    c->allocs.push_back({v, n * sizeof(T)});
     *p = static_cast<T *>(v);
     return 0;
 }
 
+// A device buffer of at least n elements at *p: the existing one when it is
+// large enough, else a new one (the old one is freed; a captured step graph
+// holding it is dropped by every caller of this function).
+template <typename T>
+int ensure(nexd_ctx *c, T **p, size_t n) {
+    if (n == 0) n = 1;
+    if (*p) {
+        for (size_t i = 0; i < c->allocs.size(); ++i) {
+            if (c->allocs[i].first != (void *)*p) continue;
+            if (c->allocs[i].second >= n * sizeof(T)) return 0;
+            HIPCHK(c, hipDeviceSynchronize());
+            (void)hipFree(c->allocs[i].first);
+            c->allocs.erase(c->allocs.begin() + (long)i);
+            break;
+        }
+        *p = nullptr;
+    }
+    return dalloc(c, p, n);
+}
+
 template <typename T>
 int upload(nexd_ctx *c, T **p, const std::vector<T> &h) {
-    if (!*p) {
-        int rc = dalloc(c, p, h.size());
-        if (rc) return rc;
-    }
+    int rc = ensure(c, p, h.size());
+    if (rc) return rc;
     HIPCHK(c, hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
     return 0;
 }
@@ -120,52 +141,14 @@ void drop_graph(nexd_ctx *c) {
 }
 
 // ---------------------------------------------------------------------------
-// stage launches, dispatched on the compile-time order
+// stage launches (stage_launch.hpp: the kernels are instantiated in the
+// stage_*.hip translation units)
 // ---------------------------------------------------------------------------
-template <int ORDER, bool ATT, bool PML>
-void launch_one(const StageArgs<double> &a, hipStream_t s) {
-    using S = nexd::Shape<ORDER, ATT, PML, double>;
-    const int grid = (a.nlist + S::EB - 1) / S::EB * a.nshot;
-    if (grid == 0) return;
-    hipLaunchKernelGGL((nexd::stage_kernel<ORDER, ATT, PML, double>), dim3(grid), dim3(S::NTHR), 0, s, a);
-}
-
-template <int ORDER>
-void launch_stage(const StageArgs<double> &a, bool att, bool pml, hipStream_t s) {
-    if (att && pml) launch_one<ORDER, true, true>(a, s);
-    else if (att) launch_one<ORDER, true, false>(a, s);
-    else if (pml) launch_one<ORDER, false, true>(a, s);
-    else launch_one<ORDER, false, false>(a, s);
-}
-
-template <int ORDER>
-int blocks_for(int n, bool att, bool pml) {
-    int eb;
-    if (att && pml) eb = nexd::Shape<ORDER, true, true, double>::EB;
-    else if (att) eb = nexd::Shape<ORDER, true, false, double>::EB;
-    else if (pml) eb = nexd::Shape<ORDER, false, true, double>::EB;
-    else eb = nexd::Shape<ORDER, false, false, double>::EB;
-    return (n + eb - 1) / eb;
-}
-
-int nblocks(int order, int n, bool att, bool pml) {
-    switch (order) {
-        case 1: return blocks_for<1>(n, att, pml);
-        case 2: return blocks_for<2>(n, att, pml);
-        case 3: return blocks_for<3>(n, att, pml);
-        case 4: return blocks_for<4>(n, att, pml);
-        default: return blocks_for<5>(n, att, pml);
-    }
-}
+int nblocks(int order, int n, bool att, bool pml) { return nexd::stage_blocks<double>(order, n, att, pml); }
 
 void dispatch_stage(int order, const StageArgs<double> &a, bool att, bool pml, hipStream_t s) {
-    switch (order) {
-        case 1: launch_stage<1>(a, att, pml, s); break;
-        case 2: launch_stage<2>(a, att, pml, s); break;
-        case 3: launch_stage<3>(a, att, pml, s); break;
-        case 4: launch_stage<4>(a, att, pml, s); break;
-        default: launch_stage<5>(a, att, pml, s); break;
-    }
+    if (a.scheme == NEXD_LSRK4) nexd::launch_stage<double, true>(order, att, pml, a, s);
+    else nexd::launch_stage<double, false>(order, att, pml, a, s);
 }
 
 template <int ORDER>
@@ -185,9 +168,14 @@ void dispatch_last_flux(nexd_ctx *c, const double *qin, const double *pin) {
     }
 }
 
+// The reference computes the energy after every step of a run with PML
+// (par%set_pml), also after its trigger has switched the PML elements off
+// (src/timeloopMod.f90:482-516): it does not depend on pml_active.
+bool energy_on(const nexd_ctx *c) { return c->cfg.energy && c->cfg.pml && c->have_pml; }
+
 template <int ORDER>
 void launch_step_end(nexd_ctx *c, int row_offset, int advance) {
-    const double *be = (c->cfg.energy && c->pml_active && c->have_pml) ? c->blk_energy : nullptr;
+    const double *be = energy_on(c) ? c->blk_energy : nullptr;
     hipLaunchKernelGGL((nexd::step_end_kernel<ORDER, double>), dim3(1), dim3(1024), 0, c->stream, c->q[0], c->u_slot,
                        c->a_slot, c->rec_int, c->recelem, c->rec_slot_of, c->cfg.nrec, c->nslot, 1, (int)c->nglob,
                        c->cfg.nt, row_offset, advance, c->seis, be, c->nblk_total, c->energy, c->d_it);
@@ -229,7 +217,9 @@ StageArgs<double> base_args(nexd_ctx *c) {
     a.src_array = c->src_array; a.src_array_m = c->src_array_m; a.nsrc = c->cfg.nsrc;
     a.d_it = c->d_it;
     a.rec_slot = c->rec_slot; a.u_slot = c->u_slot; a.a_slot = c->a_slot; a.u_full = c->u_full;
-    a.mass = c->mass; a.jac = c->jac; a.emat = c->emat;
+    a.mass = c->mass; a.jac = c->jac; a.emat = c->emat; a.emat_stride = 0;
+    a.blk_stride = c->nblk_total;
+    a.pml_on = nullptr;
     return a;
 }
 
@@ -244,16 +234,27 @@ void stage_buffers(nexd_ctx *c, int irk, StageArgs<double> &a) {
     } else if (scheme == NEXD_RK2) {
         if (irk == 1) { qin = c->q[0]; qout = c->q[1]; thin = c->th[0]; thout = c->th[1]; pin = c->paux[0]; pout = c->paux[1]; }
         else { qin = c->q[1]; qout = c->q[0]; thin = c->th[1]; thout = c->th[0]; pin = c->paux[1]; pout = c->paux[0]; }
-    } else {
+    } else if (scheme == NEXD_RK3) {
         if (irk == 1) { qin = c->q[0]; qout = c->q[1]; thin = c->th[0]; thout = c->th[1]; pin = c->paux[0]; pout = c->paux[1]; }
         else if (irk == 2) { qin = c->q[1]; qout = c->q[2]; thin = c->th[1]; thout = c->th[1]; pin = c->paux[1]; pout = c->paux[1]; }
         else { qin = c->q[2]; qout = c->q[0]; thin = c->th[1]; thout = c->th[0]; pin = c->paux[1]; pout = c->paux[0]; }
+    } else {
+        // low-storage RK4: q rotates 0 -> 1 -> 2 -> 1 -> 2 -> 0 (the face
+        // term reads neighbours of the stage input); memory variables and
+        // PML auxiliaries, read only at their own node, update in place
+        static const int from[5] = {0, 1, 2, 1, 2}, to[5] = {1, 2, 1, 2, 0};
+        qin = c->q[from[irk - 1]]; qout = c->q[to[irk - 1]];
+        thin = thout = c->th[0]; pin = pout = c->paux[0];
     }
     a.irk = irk;
     a.last_stage = irk == c->nrk ? 1 : 0;
-    a.qin = qin; a.qbase = c->q[0]; a.qout = qout;
-    a.thin = thin; a.thbase = c->th[0]; a.thout = thout;
-    a.pin = pin; a.pbase = c->paux[0]; a.pout = pout;
+    const bool ls = scheme == NEXD_LSRK4;
+    a.qin = qin; a.qbase = ls ? c->qres : c->q[0]; a.qout = qout;
+    a.thin = thin; a.thbase = ls ? c->thres : c->th[0]; a.thout = thout;
+    a.pin = pin; a.pbase = ls ? c->pres : c->paux[0]; a.pout = pout;
+    a.qres = ls ? c->qres : nullptr; a.thres = ls ? c->thres : nullptr; a.pres = ls ? c->pres : nullptr;
+    a.lsa = ls ? nexd::kLsrkA[irk - 1] : 0.0;
+    a.lsb = ls ? nexd::kLsrkB[irk - 1] : 0.0;
 }
 
 void enqueue_step(nexd_ctx *c) {
@@ -265,7 +266,9 @@ void enqueue_step(nexd_ctx *c) {
     for (int irk = 1; irk <= nrk; ++irk) {
         stage_buffers(c, irk, a);
         const double *qin = a.qin, *pin = a.pin;
-        const bool energy = a.last_stage && c->cfg.energy && pml;
+        // energy partials: the plain list's blocks first, then the PML list's
+        // (every element is in the plain list once the PML is switched off)
+        const bool energy = a.last_stage && energy_on(c);
         // elements without PML (every element when PML is off); with PML they
         // run on a side stream, beside the PML list: both read the stage
         // input and write disjoint elements and energy partials
@@ -332,7 +335,8 @@ int nexd_create(const nexd_config *cfg, nexd_ctx **out) {
     if (cfg->order < 1 || cfg->order > 5) return fail(nullptr, NEXD_E_UNSUPPORTED, "order must be 1..5");
     if (cfg->nelem <= 0 || cfg->nt <= 0 || cfg->nsrc < 0 || cfg->nrec < 0)
         return fail(nullptr, NEXD_E_ARG, "nelem, nt must be positive; nsrc, nrec non-negative");
-    if (cfg->timeint < 1 || cfg->timeint > 3) return fail(nullptr, NEXD_E_UNSUPPORTED, "timeint must be 1, 2 or 3");
+    if (cfg->timeint < 1 || cfg->timeint > 4)
+        return fail(nullptr, NEXD_E_UNSUPPORTED, "timeint must be 1, 2, 3 or 4 (low-storage RK4)");
     if (!(cfg->dt > 0)) return fail(nullptr, NEXD_E_ARG, "dt must be positive");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(nullptr, NEXD_E_HIP, "no HIP device");
@@ -343,7 +347,7 @@ int nexd_create(const nexd_config *cfg, nexd_ctx **out) {
     c->npf = (cfg->order + 1) * (cfg->order + 2) / 2;
     c->nfp = 4 * c->npf;
     c->nglob = (size_t)cfg->nelem * c->np;
-    c->nrk = cfg->timeint == 1 ? 1 : (cfg->timeint == 2 ? 2 : 3);
+    c->nrk = cfg->timeint == 1 ? 1 : (cfg->timeint == 2 ? 2 : (cfg->timeint == 3 ? 3 : 5));
     auto bail = [&](int rc) { g_create_error = c->err; nexd_destroy(c); return rc; };
     if (hipSetDevice(cfg->device) != hipSuccess) return bail(fail(c, NEXD_E_HIP, "hipSetDevice failed"));
     if (hipStreamCreate(&c->stream) != hipSuccess || hipStreamCreate(&c->side) != hipSuccess)
@@ -360,6 +364,11 @@ int nexd_create(const nexd_config *cfg, nexd_ctx **out) {
         (rc = dalloc(c, &c->paux[0], 27 * ng)) || (rc = dalloc(c, &c->paux[1], 27 * ng)) ||
             (rc = dalloc(c, &c->pmlc, 9 * ng)) || (rc = dalloc(c, &c->last_flux, 27 * c->np));
     if (!rc && cfg->track_displacement) rc = dalloc(c, &c->u_full, 3 * ng);
+    if (!rc && cfg->timeint == NEXD_LSRK4) {
+        rc = dalloc(c, &c->qres, 9 * ng);
+        if (!rc && cfg->attenuation) rc = dalloc(c, &c->thres, 18 * ng);
+        if (!rc && cfg->pml) rc = dalloc(c, &c->pres, 27 * ng);
+    }
     if (!rc)
         (rc = dalloc(c, &c->elist_plain, cfg->nelem)) || (rc = dalloc(c, &c->elist_pml, cfg->nelem)) ||
             (rc = dalloc(c, &c->d_it, 1)) ||
@@ -515,10 +524,12 @@ int nexd_upload_receivers(nexd_ctx *c, const int *recelem, const double *rec_int
     }
     std::vector<double> w(rec_int, rec_int + (size_t)np * nr);
     int rc;
-    if (!c->u_slot) {
-        if ((rc = dalloc(c, &c->u_slot, (size_t)nslot * 3 * np)) || (rc = dalloc(c, &c->a_slot, (size_t)nslot * 3 * np)))
-            return rc;
-    }
+    // the receiver-element slots carry state (displacement) from step to
+    // step: they can change only before the first step
+    if (c->step > 0) return fail(c, NEXD_E_STATE, "receivers cannot change after steps have run (reset first)");
+    if ((rc = ensure(c, &c->u_slot, (size_t)nslot * 3 * np)) || (rc = ensure(c, &c->a_slot, (size_t)nslot * 3 * np)))
+        return rc;
+    c->initialized = false;   // the slots get the initial state at the next run
     c->nslot = nslot;
     if ((rc = upload(c, &c->rec_slot, slot_of_elem)) || (rc = upload(c, &c->rec_slot_of, slot_of_rec)) ||
         (rc = upload(c, &c->recelem, re)) || (rc = upload(c, &c->rec_int, w)))
@@ -544,6 +555,12 @@ int nexd_reset(nexd_ctx *c) {
         for (int i = 0; i < 2; ++i) HIPCHK(c, hipMemcpy(c->paux[i], z.data(), sizeof(double) * 27 * ng, hipMemcpyHostToDevice));
     }
     if (c->u_full) HIPCHK(c, hipMemcpy(c->u_full, tiny.data(), sizeof(double) * 3 * ng, hipMemcpyHostToDevice));
+    if (c->qres) {
+        const std::vector<double> z(27 * ng, 0.0);
+        HIPCHK(c, hipMemcpy(c->qres, z.data(), sizeof(double) * 9 * ng, hipMemcpyHostToDevice));
+        if (c->thres) HIPCHK(c, hipMemcpy(c->thres, z.data(), sizeof(double) * 18 * ng, hipMemcpyHostToDevice));
+        if (c->pres) HIPCHK(c, hipMemcpy(c->pres, z.data(), sizeof(double) * 27 * ng, hipMemcpyHostToDevice));
+    }
     if (c->nslot) {
         std::vector<double> t(3 * (size_t)c->nslot * c->np, tiny_v);
         HIPCHK(c, hipMemcpy(c->u_slot, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice));
@@ -588,7 +605,7 @@ int nexd_run(nexd_ctx *c, int it_begin, int it_end, double *energy) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
-    if (energy && c->cfg.energy && c->cfg.pml)
+    if (energy && energy_on(c))
         HIPCHK(c, hipMemcpy(energy, c->energy + (it_begin - 1), sizeof(double) * (it_end - it_begin + 1), hipMemcpyDeviceToHost));
     c->step = it_end;
     return NEXD_OK;
@@ -596,6 +613,7 @@ int nexd_run(nexd_ctx *c, int it_begin, int it_end, double *energy) {
 
 int nexd_set_pml_active(nexd_ctx *c, int active) {
     if (!c) return fail(c, NEXD_E_ARG, "null context");
+    if (!c->cfg.pml && active) return fail(c, NEXD_E_STATE, "context created without PML");
     c->pml_active = active != 0;
     return rebuild_lists(c);
 }
@@ -698,8 +716,12 @@ int nexd_time_stage_kernel(nexd_ctx *c, int list, int nrep, double *avg_ms, int 
     // it.  The last element's fluxes are computed once, before the timed
     // launches: their values do not change the work.
     const bool att = c->cfg.attenuation != 0;
+    // the last stage writes its energy partials, as in the time loop
+    double *en = energy_on(c) ? c->blk_energy + (list == 1 ? nblocks(c->cfg.order, c->n_plain, att, false) : 0)
+                              : nullptr;
     auto one = [&](int irk) {
         stage_buffers(c, irk, a);
+        a.blk_energy = a.last_stage ? en : nullptr;
         dispatch_stage(c->cfg.order, a, att, list == 1, c->stream);
     };
     stage_buffers(c, 1, a);
@@ -723,7 +745,7 @@ void nexd_destroy(nexd_ctx *c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graph(c);
-    for (void *p : c->allocs) (void)hipFree(p);
+    for (auto &a : c->allocs) (void)hipFree(a.first);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);

@@ -52,12 +52,13 @@ struct StageArgs {
     const int *elist;           // elements handled by this launch
     int nlist;
     int irk;                    // 1-based stage
-    int scheme;                 // 1 Euler, 2 RK2, 3 RK3
+    int scheme;                 // 1 Euler, 2 RK2, 3 RK3, 4 low-storage RK4
+    R lsa, lsb;                 // low-storage RK4: A and B of this stage
     int last_stage;             // 1 on the stage that completes a step
     R dt;
     R w13, w23;                 // RK3 final-stage weights
     const R *qin;               // [9][nglob] stage input
-    const R *qbase;             // [9][nglob] step-start state (RK base)
+    const R *qbase;             // [9][nglob] step-start state (RK base); low-storage RK4: the residual
     R *qout;                    // [9][nglob]
     const R *thin;              // [18][nglob]
     const R *thbase;
@@ -94,16 +95,22 @@ struct StageArgs {
     R *u_slot;                  // [shot][nslot][3][NP]
     R *a_slot;                  // [shot][nslot][3][NP]
     R *u_full;                  // [3][nglob] or null
-    R *blk_energy;              // [gridDim] or null
+    double *blk_energy;         // [shot][blk_stride] energy partials (this list's part) or null
+    int blk_stride;             // per-shot stride of blk_energy: element blocks of a whole step
     const R *mass;              // [NP][NP]
     const R *jac;               // [nglob]
     const R *emat;              // [nelem][2] mu, lambda for the energy
+    int emat_stride;            // per-shot stride of emat (0: shared)
+    const int *pml_on;          // [shot] 0 once a shot's PML is switched off; null: on
     // compensated state (float pairs hi + lo for the 9 wavefield components
     // and the receiver-element displacement); null when not compensated
     const R *qlo_in;            // [9][nglob] low parts of qin
     const R *qlo_base;          // [9][nglob] low parts of qbase
     R *qlo_out;                 // [9][nglob]
     R *u_slot_lo;               // [shot][nslot][3][NP]
+    // low-storage RK4: the residual buffers, updated in place (the same
+    // memory as qbase, thbase, pbase); null for the other schemes
+    R *qres, *thres, *pres;
 };
 
 // ---------------------------------------------------------------------------
@@ -170,6 +177,10 @@ __device__ __forceinline__ void upwind_flux(const R *fr, R vp, R vs, R rho, R la
 //   TVD RK2, stage 2:      ((base + a) + dt*rhs) / 2
 //   TVD RK3, stage 2:      3/4 base + 1/4 (a + dt*rhs)
 //   TVD RK3, stage 3:      w13 base + w23 (a + dt*rhs)
+// and the five-stage low-storage RK4 of Carpenter and Kennedy (NASA
+// TM-109112, 1994; no counterpart in the reference), whose base is the
+// residual of the previous stage:
+//   res_new = A_k res + dt*rhs  (A_1 = 0),   a_new = a + B_k res_new
 // The stage is uniform over a launch, so its weights are chosen once
 // (rk_stage) and every combined value is the same straight-line code: both
 // forms are computed and one is selected.  A branch per value would split a
@@ -185,17 +196,28 @@ __device__ __forceinline__ void upwind_flux(const R *fr, R vp, R vs, R rho, R la
 __device__ __forceinline__ float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_r(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
+// Carpenter and Kennedy's five-stage fourth-order 2N-storage coefficients
+// (NASA TM-109112, 1994), as published
+constexpr double kLsrkA[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                              -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+constexpr double kLsrkB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                              1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                              2277821191437.0 / 14882151754819.0};
+
 template <typename R>
 struct RkStage {
     R wb, wx;    // wb * base + wx * (a + dt*rhs)
     bool avg;    // TVD RK2 stage 2
+    bool ls;     // low-storage RK4
+    R la, lb;    // its A and B of the stage
 };
 template <typename R>
-__device__ __forceinline__ RkStage<R> rk_stage(int scheme, int irk, R w13, R w23) {
-    if (irk == 1) return {R(0), R(1), false};
-    if (scheme == 2) return {R(0.5), R(0.5), true};
-    if (irk == 2) return {R(0.75), R(0.25), false};
-    return {w13, w23, false};
+__device__ __forceinline__ RkStage<R> rk_stage(int scheme, int irk, R w13, R w23, R lsa, R lsb) {
+    if (scheme == 4) return {R(0), R(1), false, true, irk == 1 ? R(0) : lsa, lsb};
+    if (irk == 1) return {R(0), R(1), false, false, R(0), R(0)};
+    if (scheme == 2) return {R(0.5), R(0.5), true, false, R(0), R(0)};
+    if (irk == 2) return {R(0.75), R(0.25), false, false, R(0), R(0)};
+    return {w13, w23, false, false, R(0), R(0)};
 }
 template <typename R, bool WEIGHTED = false>
 __device__ __forceinline__ R rk_combine(const RkStage<R> &s, R a, R base, R rhs, R dt) {
@@ -206,6 +228,21 @@ __device__ __forceinline__ R rk_combine(const RkStage<R> &s, R a, R base, R rhs,
     if constexpr (WEIGHTED) return z;
     const R y = ((base + a) + k) * R(0.5);
     return s.avg ? y : z;
+}
+// One stage of either kind: the low-storage form when LS (`res` is the
+// previous residual on entry and the new one on return; A_1 = 0, so the
+// stage-1 residual need only be finite), rk_combine otherwise (`res` is the
+// RK base and stays as it is).
+template <typename R, bool LS, bool WEIGHTED = false>
+__device__ __forceinline__ R rk_update(const RkStage<R> &s, R a, R &res, R rhs, R dt) {
+#pragma clang fp contract(off)
+    if constexpr (LS) {
+        const R r = s.la * res + rhs * dt;
+        res = r;
+        return a + s.lb * r;
+    } else {
+        return rk_combine<R, WEIGHTED>(s, a, res, rhs, dt);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -412,7 +449,9 @@ struct Mfma<double> {
     }
 };
 
-template <int ORDER, bool ATT, bool PML, typename R>
+// LS: the low-storage RK4 instantiation (residuals read and written in
+// place of the RK base); the TVD schemes use LS = false.
+template <int ORDER, bool ATT, bool PML, typename R, bool LS>
 __global__ void __launch_bounds__((Shape<ORDER, ATT, PML, R>::NTHR), (Shape<ORDER, ATT, PML, R>::WPS))
 stage_kernel(StageArgs<R> a) {
     using S = Shape<ORDER, ATT, PML, R>;
@@ -429,9 +468,7 @@ stage_kernel(StageArgs<R> a) {
     R *sV = lds;                          // [NT][NV]    volume values, after the volume product
     R *sM = lds + S::OFF_M;               // [9][NT]     metric of the block's nodes
     R *sL = lds + S::OFF_L;               // [16][SLN]   lifted face values, node rows
-    constexpr int NRED = NTHR <= 64 ? 64 : (NTHR <= 128 ? 128 : 256);   // power of two >= NTHR
-    static_assert(NRED == NTHR, "the block reduction assumes a power-of-two block");
-    R *sRed = lds + 15 * NT;              // [NTHR]      energy reduction
+    static_assert(NTHR % 64 == 0, "the energy block sum assumes whole waves");
 
     const int t = threadIdx.x;
     const size_t ng = (size_t)a.nglob;
@@ -543,7 +580,12 @@ stage_kernel(StageArgs<R> a) {
             // stage-start values, so it is done here and the auxiliaries die
             R Fm[9], Gm[9], Hm[9];
             elastic_flux_node(qs, ef0, ef1, ef2, ef3, Fm, Gm, Hm);
-            const RkStage<R> rs = rk_stage<R>(a.scheme, a.irk, a.w13, a.w23);
+            const RkStage<R> rs = rk_stage<R>(a.scheme, a.irk, a.w13, a.w23, a.lsa, a.lsb);
+            // a shot whose PML is switched off (src/timeloopMod.f90:532-539)
+            // keeps its PML elements in this launch with the modification
+            // and the ADE update made neutral: auxiliaries read as 0,
+            // alpha = d/k = 0, 1/k = 1, so the fluxes are the elastic ones
+            const bool on = !a.pml_on || a.pml_on[shot] != 0;
             // F_4, G_5 and H_3 are identically zero (src/waveMod.f90:37-62) and
             // left out of the volume columns.  Contravariant form: C_o,i = o_x
             // F_i + o_y G_i + o_z H_i, o = r, s, t with the node's metric
@@ -553,16 +595,16 @@ stage_kernel(StageArgs<R> a) {
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 R pv[9];
-                const R al = pc[d], dk = pc[3 + d];
+                const R al = on ? pc[d] : R(0), dk = on ? pc[3 + d] : R(0);
                 // (f' + F) / k as a product with 1/k: one division per
                 // direction instead of nine, within an ulp of the quotient
-                const R rk = R(1) / pc[6 + d];
+                const R rk = on ? R(1) / pc[6 + d] : R(1);
                 R *FF = d == 0 ? Fm : (d == 1 ? Gm : Hm);
                 const int skip = d == 0 ? 4 : (d == 1 ? 5 : 3);
 #pragma unroll
                 for (int i = 0; i < 9; ++i) {
                     const int c = d * 9 + i;
-                    const R c0 = cur[c];
+                    const R c0 = on ? cur[c] : R(0);
                     const R mod = (c0 + FF[i]) * rk;
                     if constexpr (KC) {
 #pragma unroll
@@ -579,13 +621,20 @@ stage_kernel(StageArgs<R> a) {
                     const R lastv = sLast[c * NP + j];
                     const R u = a.pml_compat ? lastv : mod;
                     const R dadt = -al * c0 - dk * (c0 + u);
-                    pv[i] = rk_combine<R>(rs, c0, bse[c], dadt, a.dt);
+                    pv[i] = rk_update<R, LS>(rs, c0, bse[c], dadt, a.dt);
                 }
                 // one guarded block of stores per direction (a guard per
                 // store leaves waits for every earlier store at the joins)
                 if (valid) {
 #pragma unroll
                     for (int i = 0; i < 9; ++i) pout[(d * 9 + i) * ng + g] = pv[i];
+                }
+                if constexpr (LS) {
+                    if (valid) {
+                        R *pres = a.pres + (size_t)shot * 27 * ng;
+#pragma unroll
+                        for (int i = 0; i < 9; ++i) pres[(d * 9 + i) * ng + g] = bse[d * 9 + i];
+                    }
                 }
                 // the nine updates of a direction interleave; a scheduling
                 // fence per direction keeps the working set in registers
@@ -1020,7 +1069,7 @@ stage_kernel(StageArgs<R> a) {
                 exy = gv[3] + gv[1]; eyz = gv[7] + gv[5]; exz = gv[6] + gv[2];
             }
             R *thout = a.thout + (size_t)shot * 18 * ng;
-            const RkStage<R> rs = rk_stage<R>(a.scheme, a.irk, a.w13, a.w23);
+            const RkStage<R> rs = rk_stage<R>(a.scheme, a.irk, a.w13, a.w23, a.lsa, a.lsb);
 #pragma unroll
             for (int i = 0; i < NMB; ++i) {
                 const R w = a.wl[i];
@@ -1038,13 +1087,20 @@ stage_kernel(StageArgs<R> a) {
                 for (int c = 0; c < 6; ++c) {
                     const int k = i * 6 + c;
                     const R arq = vol[c] + w * lf[9 + c] - w * th[k];
-                    th[k] = rk_combine<R, sizeof(R) == 4>(rs, th[k], thb[k], arq, a.dt);
+                    th[k] = rk_update<R, LS, sizeof(R) == 4>(rs, th[k], thb[k], arq, a.dt);
                 }
             }
             // one guarded block of stores
             if (valid) {
 #pragma unroll
                 for (int k = 0; k < 18; ++k) thout[k * ng + g] = th[k];
+            }
+            if constexpr (LS) {
+                if (valid) {
+                    R *thres = a.thres + (size_t)shot * 18 * ng;
+#pragma unroll
+                    for (int k = 0; k < 18; ++k) thres[k * ng + g] = thb[k];
+                }
             }
         }
     }
@@ -1079,7 +1135,7 @@ stage_kernel(StageArgs<R> a) {
             }
         }
         // Runge-Kutta update
-        if (a.qlo_out) {
+        if (!LS && a.qlo_out) {
             const size_t so = (size_t)shot * 9 * ng;
             R alo[9], blo[9];
 #pragma unroll
@@ -1100,12 +1156,19 @@ stage_kernel(StageArgs<R> a) {
                 if (valid) { qout[f * ng + g] = qnew[f]; a.qlo_out[so + f * ng + g] = lo; }
             }
         } else {
-            const RkStage<R> rs = rk_stage<R>(a.scheme, a.irk, a.w13, a.w23);
+            const RkStage<R> rs = rk_stage<R>(a.scheme, a.irk, a.w13, a.w23, a.lsa, a.lsb);
 #pragma unroll
-            for (int f = 0; f < 9; ++f) qnew[f] = rk_combine<R, sizeof(R) == 4>(rs, qs[f], qb[f], rhs[f], a.dt);
+            for (int f = 0; f < 9; ++f) qnew[f] = rk_update<R, LS, sizeof(R) == 4>(rs, qs[f], qb[f], rhs[f], a.dt);
             if (valid) {
 #pragma unroll
                 for (int f = 0; f < 9; ++f) qout[f * ng + g] = qnew[f];
+            }
+            if constexpr (LS) {
+                if (valid) {
+                    R *qres = a.qres + (size_t)shot * 9 * ng;
+#pragma unroll
+                    for (int f = 0; f < 9; ++f) qres[f * ng + g] = qb[f];
+                }
             }
         }
     }
@@ -1137,26 +1200,38 @@ stage_kernel(StageArgs<R> a) {
             }
         }
         if (a.blk_energy) {
-            // energy partial of this block (src/timeloopMod.f90:490-516)
+            // energy partial of this block (src/timeloopMod.f90:490-516), summed
+            // in FP64.  The reference's normal "strains" are stress combinations
+            // divided by temp1 = 1/(4 mu^2 + 6 mu lam), that is multiplied by
+            // ce = 4 mu^2 + 6 mu lam; with stresses of a few 1e10 the products
+            // leave the FP32 range, so FP32 carries the combinations through the
+            // mass product and applies ce in FP64.  FP64 follows the reference.
+            constexpr bool F32 = sizeof(R) == 4;
             __syncthreads();
             R *sE = lds;   // [15][NT]
             R lam_e = 0, mu_e = 0;
-            if (valid) { mu_e = a.emat[(size_t)e * 2]; lam_e = a.emat[(size_t)e * 2 + 1]; }
+            if (valid) {
+                const R *em = a.emat + (size_t)shot * a.emat_stride + (size_t)e * 2;
+                mu_e = em[0]; lam_e = em[1];
+            }
             if (node_thr) {
                 const R temp1 = R(1.) / (4 * mu_e * mu_e + 6 * mu_e * lam_e);
                 const R temp2 = 2 * mu_e;
                 R y[15];
 #pragma unroll
                 for (int c = 0; c < 9; ++c) y[c] = qnew[c];
-                y[9] = (2 * (lam_e + mu_e) * qnew[0] - lam_e * (qnew[1] + qnew[2])) / temp1;
-                y[10] = (2 * (lam_e + mu_e) * qnew[1] - lam_e * (qnew[0] + qnew[2])) / temp1;
-                y[11] = (2 * (lam_e + mu_e) * qnew[2] - lam_e * (qnew[0] + qnew[0])) / temp1;
+                const R c0 = 2 * (lam_e + mu_e) * qnew[0] - lam_e * (qnew[1] + qnew[2]);
+                const R c1 = 2 * (lam_e + mu_e) * qnew[1] - lam_e * (qnew[0] + qnew[2]);
+                const R c2 = 2 * (lam_e + mu_e) * qnew[2] - lam_e * (qnew[0] + qnew[0]);
+                y[9] = F32 ? c0 : c0 / temp1;
+                y[10] = F32 ? c1 : c1 / temp1;
+                y[11] = F32 ? c2 : c2 / temp1;
                 y[12] = qnew[3] / temp2; y[13] = qnew[4] / temp2; y[14] = qnew[5] / temp2;
 #pragma unroll
                 for (int c = 0; c < 15; ++c) sE[c * NT + t] = valid ? y[c] : R(0);
             }
             __syncthreads();
-            R part = 0;
+            double part = 0;
             if (valid) {
                 R mv[9];
 #pragma unroll
@@ -1169,20 +1244,34 @@ stage_kernel(StageArgs<R> a) {
 #pragma unroll
                     for (int c = 0; c < 6; ++c) mv[c] += mjk * sE[(9 + c) * NT + ebase + k];
                 }
-                const R jac0 = a.jac[(size_t)e * NP];
-                const R d = rho * (qnew[6] * mv[6]) + rho * (qnew[7] * mv[7]) + rho * (qnew[8] * mv[8]) +
-                            qnew[0] * mv[0] + qnew[1] * mv[1] + qnew[2] * mv[2] +
-                            2 * (qnew[3] * mv[3] + qnew[4] * mv[4] + qnew[5] * mv[5]);
-                part = R(0.5) * jac0 * d;
+                const double jac0 = a.jac[(size_t)e * NP];
+                double d;
+                if constexpr (!F32) {
+                    d = rho * (qnew[6] * mv[6]) + rho * (qnew[7] * mv[7]) + rho * (qnew[8] * mv[8]) +
+                        qnew[0] * mv[0] + qnew[1] * mv[1] + qnew[2] * mv[2] +
+                        2 * (qnew[3] * mv[3] + qnew[4] * mv[4] + qnew[5] * mv[5]);
+                } else {
+                    // the three groups in FP32 (each within its range), combined in FP64
+                    const R dv = qnew[6] * mv[6] + qnew[7] * mv[7] + qnew[8] * mv[8];
+                    const R dn = qnew[0] * mv[0] + qnew[1] * mv[1] + qnew[2] * mv[2];
+                    const R ds = qnew[3] * mv[3] + qnew[4] * mv[4] + qnew[5] * mv[5];
+                    const double mu = mu_e, lam = lam_e;
+                    d = (double)rho * dv + (4 * mu * mu + 6 * mu * lam) * dn + 2.0 * ds;
+                }
+                part = 0.5 * jac0 * d;
             }
-            sRed[t] = part;
-            if (t + NTHR < NRED) sRed[t + NTHR] = 0;
+            // block sum in a fixed order: within each wave, then over the waves
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off, 64);
+            __shared__ double sWave[NTHR / 64];
+            if ((t & 63) == 0) sWave[t >> 6] = part;
             __syncthreads();
-            for (int s = NRED / 2; s > 0; s >>= 1) {
-                if (t < s) sRed[t] += sRed[t + s];
-                __syncthreads();
+            if (t == 0) {
+                double sum = 0;
+#pragma unroll
+                for (int i = 0; i < NTHR / 64; ++i) sum += sWave[i];
+                a.blk_energy[(size_t)shot * a.blk_stride + eblk] = sum;
             }
-            if (t == 0) a.blk_energy[blockIdx.x] = sRed[0];
         }
     }
 }
@@ -1227,16 +1316,16 @@ __global__ void last_flux_kernel(const R *qin, const R *pin, const R *pmlc, cons
 }
 
 // Receiver sampling (src/timeloopMod.f90:358-373) of row `row_offset + *d_it`
-// for every shot, the energy of the step (sum of the stage kernels' block
-// partials in a fixed order), and the step-counter advance.  One block.
+// for every shot, each shot's energy of the step (sum of the stage kernels'
+// block partials in a fixed order), and the step-counter advance.  One block.
 template <int ORDER, typename R>
 __global__ void __launch_bounds__(1024)
 step_end_kernel(const R *q, const R *u_slot, const R *a_slot, const R *rec_int, const int *recelem,
                 const int *rec_slot_of, int nrec, int nslot, int nshot, int nglob, int nt, int row_offset,
-                int advance, R *seis, const R *blk_energy, int nblk, R *energy_out, int *d_it) {
+                int advance, R *seis, const double *blk_energy, int nblk, double *energy_out, int *d_it) {
     constexpr int NP = Dims<ORDER>::NP;
     __shared__ int s_it;
-    __shared__ R s_red[1024];
+    __shared__ double s_red[1024];
     if (threadIdx.x == 0) s_it = *d_it;
     __syncthreads();
     const int it = s_it;
@@ -1262,15 +1351,20 @@ step_end_kernel(const R *q, const R *u_slot, const R *a_slot, const R *rec_int, 
         }
     }
     if (advance && blk_energy) {
-        R acc = 0;
-        for (int b = threadIdx.x; b < nblk; b += blockDim.x) acc += blk_energy[b];
-        s_red[threadIdx.x] = acc;
-        __syncthreads();
-        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) s_red[threadIdx.x] += s_red[threadIdx.x + s];
+        // energy_out[shot][nt]: each shot's partials [shot][nblk] in a fixed order
+        for (int shot = 0; shot < nshot; ++shot) {
+            const double *pb = blk_energy + (size_t)shot * nblk;
+            double acc = 0;
+            for (int b = threadIdx.x; b < nblk; b += blockDim.x) acc += pb[b];
+            s_red[threadIdx.x] = acc;
+            __syncthreads();
+            for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+                if ((int)threadIdx.x < s) s_red[threadIdx.x] += s_red[threadIdx.x + s];
+                __syncthreads();
+            }
+            if (threadIdx.x == 0 && it < nt) energy_out[(size_t)shot * nt + it] = s_red[0];
             __syncthreads();
         }
-        if (threadIdx.x == 0) energy_out[it] = s_red[0];
     }
     __syncthreads();
     if (advance && threadIdx.x == 0) *d_it = it + 1;

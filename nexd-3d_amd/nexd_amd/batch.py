@@ -69,7 +69,9 @@ def _dtfactor(base: problem.Problem, vpu) -> float:
     return min(1e7, float(np.min(meshmod.F32_2_3 * base.ref.min_gll * (rtemp1 / vpu))))
 
 
-def _finish(base: problem.Problem, shots, shared: bool, full_source_arrays: bool = False) -> ShotBatch:
+def finish_shots(base: problem.Problem, shots, shared: bool, full_source_arrays: bool = False) -> ShotBatch:
+    """The shots as one batch: one time step for all (the smallest CFL step
+    of their materials) and each shot's source operands at that step."""
     dt = min(_dtfactor(base, s.mat.vpu) for s in shots) * base.par["cfl"]
     out = []
     for s in shots:
@@ -105,7 +107,7 @@ def multishot_cube(nshot: int, n: int = 8, order: int = 3, nt: int = 4000, exten
         xyz = tuple(float(v) for v in rng.uniform(0.25 * extent, 0.75 * extent, 3))
         src = srcmod.Source(xyz=xyz, kind=1, stf=2, f0=f0, factor=1.0, M=random_moment_tensor(rng))
         shots.append(_shot_problem(base, vp, vs, base_mesh.rho.copy(), src))
-    return _finish(base, shots, shared=False, full_source_arrays=full_source_arrays)
+    return finish_shots(base, shots, shared=False, full_source_arrays=full_source_arrays)
 
 
 def layered_shots(nshot: int, seed: int = 20, full_source_arrays: bool = False, **kw) -> ShotBatch:
@@ -124,7 +126,7 @@ def layered_shots(nshot: int, seed: int = 20, full_source_arrays: bool = False, 
         src = srcmod.Source(xyz=xyz, kind=1, stf=1, f0=2.0, factor=1.0, M=random_moment_tensor(rng))
         loc = srcmod.locate([xyz], base.mesh, base.geom, base.order)
         shots.append(dataclasses.replace(base, sources=[src], src=None, extra={"loc": loc}))
-    return _finish(base, shots, shared=True, full_source_arrays=full_source_arrays)
+    return finish_shots(base, shots, shared=True, full_source_arrays=full_source_arrays)
 
 
 def uneven_instances(count: int, seed: int = 4000, n_lo: int = 10, n_hi: int = 30, orders=(2, 3, 4, 5),

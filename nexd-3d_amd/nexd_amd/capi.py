@@ -201,7 +201,8 @@ NEXD_FP64, NEXD_FP32 = 0, 1
 BATCH_EXPORTS = ["nexd_batch_create", "nexd_batch_upload_operators", "nexd_batch_upload_mesh",
                  "nexd_batch_upload_material", "nexd_batch_upload_anelastic", "nexd_batch_upload_pml",
                  "nexd_batch_upload_sources", "nexd_batch_upload_receivers", "nexd_batch_reset", "nexd_batch_run",
-                 "nexd_batch_get_seismograms", "nexd_batch_last_run_ms", "nexd_batch_time_stage_kernel",
+                 "nexd_batch_get_seismograms", "nexd_batch_get_energy", "nexd_batch_set_pml_active",
+                 "nexd_batch_last_run_ms", "nexd_batch_time_stage_kernel",
                  "nexd_batch_destroy", "nexd_batch_last_error"]
 GROUP_EXPORTS = ["nexd_group_create", "nexd_group_run", "nexd_group_last_run_ms", "nexd_group_destroy",
                  "nexd_group_last_error"]
@@ -210,7 +211,7 @@ GROUP_EXPORTS = ["nexd_group_create", "nexd_group_run", "nexd_group_last_run_ms"
 class NexdBatchConfig(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in ("nelem", "order", "nshot", "nsrc", "nrec", "nt", "timeint",
                                             "attenuation", "pml", "compat", "precision", "shared_material",
-                                            "compensated", "device")] + [("dt", ctypes.c_double)]
+                                            "compensated", "energy", "device")] + [("dt", ctypes.c_double)]
 
 
 def _load_batch():
@@ -232,7 +233,7 @@ class BatchContext:
     """Many shots on one mesh (nexd_amd.batch.ShotBatch), FP32 or FP64."""
 
     def __init__(self, sb, precision: int = NEXD_FP32, compat: int = COMPAT_ALL, device: int = 0,
-                 compensated: bool = False):
+                 compensated: bool = False, energy: bool = False):
         from . import layout
         self.lib = _load_batch()
         base = sb.base
@@ -242,7 +243,7 @@ class BatchContext:
                               nrec=a["nrec"], nt=a["nt"], timeint=a["timeint"], attenuation=a["attenuation"],
                               pml=a["pml"], compat=compat, precision=precision,
                               shared_material=int(sb.shared_material), compensated=int(compensated),
-                              device=device, dt=base.dt)
+                              energy=int(energy), device=device, dt=base.dt)
         h = ctypes.c_void_p()
         rc = self.lib.nexd_batch_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != 0:
@@ -257,7 +258,8 @@ class BatchContext:
         nsets = 1 if sb.shared_material else sb.nshot
         for k in range(nsets):
             ak = layout.reference_arrays(sb.shots[k]) if k else layout.reference_arrays(sb.shots[0])
-            self._chk(L.nexd_batch_upload_material(h, k, *[_d(ak[x]) for x in ("vpu", "vsu", "rho", "lambdau", "muu")]),
+            self._chk(L.nexd_batch_upload_material(h, k, *[_d(ak[x]) for x in ("vpu", "vsu", "rho", "lambdau", "muu",
+                                                                               "mu", "lambda")]),
                       "upload_material")
             if ak["attenuation"]:
                 self._chk(L.nexd_batch_upload_anelastic(h, k, _d(ak["wl"]), _d(ak["anelasticvar"])), "upload_anelastic")
@@ -289,6 +291,15 @@ class BatchContext:
         outs = [np.zeros(self.nrec * self.nt) for _ in range(9)]
         self._chk(self.lib.nexd_batch_get_seismograms(self.h, shot, *[_d(o) for o in outs]), "get_seismograms")
         return np.stack([o.reshape(self.nt, self.nrec).T for o in outs], axis=1)
+
+    def energy(self, shot: int) -> np.ndarray:
+        """all_energy of every step of one shot (nt,)."""
+        out = np.zeros(self.nt)
+        self._chk(self.lib.nexd_batch_get_energy(self.h, shot, _d(out)), "get_energy")
+        return out
+
+    def set_pml_active(self, shot: int, active: bool):
+        self._chk(self.lib.nexd_batch_set_pml_active(self.h, shot, int(active)), "set_pml_active")
 
     def last_run_ms(self) -> float:
         ms = ctypes.c_double()

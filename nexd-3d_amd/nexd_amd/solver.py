@@ -134,16 +134,86 @@ def setup_directory(dirname: str, order: int) -> problem.Problem:
     return p
 
 
+def trigger_start(p: problem.Problem) -> int:
+    """timecrit = 2.4/f0/deltat, the last step the trigger does not check
+    (src/timeloopMod.f90:203): the default-REAL literal 2.4 over the highest
+    source frequency (src/timeloopMod.f90:143-151) and the step, truncated
+    to an integer.  Without a source f0 is 0 and the trigger never checks."""
+    f0 = max((s.f0 for s in p.sources), default=0.0)
+    if not f0 > 0:
+        return p.nt
+    return int(float(np.float32(2.4)) / f0 / p.dt)
+
+
+def sta_lta_fires(energy: np.ndarray, it: int, par: dict, timecrit: int) -> bool:
+    """The STA/LTA trigger after step ``it`` (1-based), with energy[k] =
+    all_energy(k+1) (src/timeloopMod.f90:517-540): for it > timecrit, the
+    averages of the par%avg_window1 (long) and par%avg_window2 (short)
+    energies before step it, and the switch when |short/long - 1| exceeds
+    par%sta_lta_trigger.  The reference divides the window sums by locals
+    avg_window1/avg_window2 that it never assigns (src/timeloopMod.f90:78,
+    522, 528); the window lengths of the parameter file are used here."""
+    if it <= timecrit:
+        return False
+    w1, w2 = int(par["avg_window1"]), int(par["avg_window2"])
+    long_avg = np.float64(0.0)
+    for i in range(1, w1 + 1):
+        if it - i > 0:
+            long_avg = long_avg + np.float64(energy[it - i - 1]) / np.float64(w1)
+    short_avg = np.float64(0.0)
+    for i in range(1, w2 + 1):
+        if it - i > 0:
+            short_avg = short_avg + np.float64(energy[it - i - 1]) / np.float64(w2)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        sta_lta = short_avg / long_avg
+    return bool(abs(sta_lta - 1) > par["sta_lta_trigger"])
+
+
+def advance_with_trigger(ctx: capi.Context, p: problem.Problem):
+    """All nt steps of a run with PML and the reference's energy check:
+    the energy of every step and, once the trigger fires after step k, the
+    PML switched off for steps k+1.. (src/timeloopMod.f90:508-540).  Steps up
+    to timecrit run as one call, later ones one call per step while the
+    trigger is armed.  Returns (energy (nt,), k or 0)."""
+    nt = p.nt
+    energy = np.zeros(nt)
+    timecrit = trigger_start(p)
+    it = 1
+    first = min(nt, timecrit)
+    if first >= 1:
+        energy[:first] = ctx.run(1, first, want_energy=True)
+        it = first + 1
+    switched = 0
+    while it <= nt:
+        if switched:
+            energy[it - 1:] = ctx.run(it, nt, want_energy=True)
+            break
+        energy[it - 1] = ctx.run(it, it, want_energy=True)[0]
+        if sta_lta_fires(energy, it, p.par, timecrit):
+            ctx.set_pml_active(False)
+            switched = it
+        it += 1
+    return energy, switched
+
+
 def run(dirname: str, order: int, device: int = 0, compat: int = capi.COMPAT_ALL) -> np.ndarray:
     """Advance the whole run on the GPU and write out/seismo.*, out/stf* and,
-    with PML, out/energy as the reference does."""
+    with PML, out/energy as the reference does; with PML the energy trigger
+    runs after every step and may switch the PML off."""
     p = setup_directory(dirname, order)
     arrs = layout.reference_arrays(p)
     outdir = os.path.join(dirname, "out")
     write_stf(outdir, p)
+    energy = None
     with capi.Context(arrs, compat=compat, device=device, energy=bool(p.set_pml)) as ctx:
         ctx.reset()
-        energy = ctx.run(1, p.nt, want_energy=bool(p.set_pml))
+        if p.set_pml:
+            energy, switched = advance_with_trigger(ctx, p)
+            p.extra["pml_switched_at"] = switched
+            if switched:
+                print(f"PML was switched off at time step {switched}", flush=True)
+        else:
+            ctx.run(1, p.nt)
         seis = ctx.seismograms()
     write_seismograms(outdir, seis, p.dt, p.extra.get("recnr"))
     if energy is not None:

@@ -10,8 +10,12 @@
  *                        :257-282 (flux matrices, anelasticvar), :357-503 (loop)
  *   fluxes               src/waveMod.f90:29-91 (elastic/anelastic fluxes),
  *                        :94-267 (A+|A|, T, T^-1), :269-405 (exact Riemann fluxes)
- *   Runge-Kutta          src/rungekuttaMod.f90:27-120
+ *   Runge-Kutta          src/rungekuttaMod.f90:27-120; timeint 4 is the
+ *                        low-storage RK4 of Carpenter and Kennedy (no
+ *                        counterpart in the reference)
  *   energy               src/timeloopMod.f90:490-516
+ *   PML switch-off       src/timeloopMod.f90:532-539 (at a given step; the
+ *                        trigger rule itself is not restated here)
  */
 #include "nexd_oracle.h"
 
@@ -132,6 +136,43 @@ static inline double rk3(double a, double an, double dadt, double dt, int irk, d
     return w13 * an + w23 * (a + dadt * dt);
 }
 
+/* Low-storage RK4 (timeint = 4; no counterpart in the reference, whose
+ * schemes are Euler, TVD RK2 and TVD RK3, src/timeloopMod.f90:230-248):
+ * the five-stage fourth-order 2N-storage scheme of Carpenter and Kennedy
+ * (NASA TM-109112, 1994), coefficients as published there.  Stage k:
+ *     res = A_k*res + dt*dydt;   y = y + B_k*res       (A_1 = 0) */
+static const double LSRK_A[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                                 -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+static const double LSRK_B[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                                 1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                                 2277821191437.0 / 14882151754819.0};
+static inline double lsrk(double y, double *res, double dydt, double dt, int irk) {
+    const double r = (irk == 1 ? 0.0 : LSRK_A[irk - 1] * *res) + dt * dydt;
+    *res = r;
+    return y + LSRK_B[irk - 1] * r;
+}
+
+/* the stage arithmetic of every scheme on the scalar ODE y' = lam*y, for
+ * checks of the schemes themselves (amplification polynomial, order) */
+int oracle_rk_scalar(int timeint, int compat, double lam, double dt, int nsteps, double y0, double *y_out) {
+    const int nrk = timeint == 1 ? 1 : (timeint == 2 ? 2 : (timeint == 3 ? 3 : 5));
+    const double w13 = (compat & 1) ? (double)(1.0f / 3.0f) : 1.0 / 3.0;
+    const double w23 = (compat & 1) ? (double)(2.0f / 3.0f) : 2.0 / 3.0;
+    if (timeint < 1 || timeint > 4) return -1;
+    double y = y0, res = 0.0;
+    for (int it = 0; it < nsteps; ++it) {
+        const double yn = y;
+        for (int irk = 1; irk <= nrk; ++irk) {
+            const double f = lam * y;
+            if (timeint == 4) y = lsrk(y, &res, f, dt, irk);
+            else if (timeint == 3) y = rk3(y, yn, f, dt, irk, w13, w23);
+            else y = rk2(y, yn, f, dt, irk);
+        }
+    }
+    *y_out = y;
+    return 0;
+}
+
 /* elastic fluxes (src/waveMod.f90:29-63) of one element, (Np,9) each */
 static void elastic_fluxes(const double *qm, size_t nglob, int base, int np, const double efv[4],
                            double *F, double *G, double *H) {
@@ -165,11 +206,11 @@ int oracle_run(const oracle_problem *p, int nsteps, double *seismo, double *ener
     const size_t nglob = (size_t)nelem * np;
     const int nfp = 4 * npf;
     const double dt = p->dt;
-    const int nrk = p->timeint == 1 ? 1 : (p->timeint == 2 ? 2 : 3);
-    const int wrk = p->timeint == 3 ? 2 : 1;
+    const int nrk = p->timeint == 1 ? 1 : (p->timeint == 2 ? 2 : (p->timeint == 3 ? 3 : 5));
+    const int wrk = p->timeint == 3 ? 2 : (p->timeint == 4 ? 4 : 1);
     const double w13 = (p->compat & 1) ? (double)(1.0f / 3.0f) : 1.0 / 3.0;
     const double w23 = (p->compat & 1) ? (double)(2.0f / 3.0f) : 2.0 / 3.0;
-    if (p->timeint < 1 || p->timeint > 3) return -1;
+    if (p->timeint < 1 || p->timeint > 4) return -1;
 
     double *q = malloc(sizeof(double) * 9 * nglob), *qn = malloc(sizeof(double) * 9 * nglob);
     double *qm = malloc(sizeof(double) * 9 * nglob), *rq = malloc(sizeof(double) * 9 * nglob);
@@ -191,6 +232,13 @@ int oracle_run(const oracle_problem *p, int nsteps, double *seismo, double *ener
     if (p->pml) {
         fp = calloc(27 * nglob, sizeof(double)); fpn = calloc(27 * nglob, sizeof(double));
         fpm = calloc(27 * nglob, sizeof(double)); epml = malloc(sizeof(double) * 6 * nglob);
+    }
+    /* low-storage RK residuals (timeint 4) */
+    double *resq = NULL, *resth = NULL, *resfp = NULL;
+    if (wrk == 4) {
+        resq = calloc(9 * nglob, sizeof(double));
+        if (p->attenuation) resth = calloc(nth, sizeof(double));
+        if (p->pml) resfp = calloc(27 * nglob, sizeof(double));
     }
     int *pmlcheck = malloc(sizeof(int) * (size_t)nelem);
     for (int e = 0; e < nelem; ++e) pmlcheck[e] = p->pml ? p->pmlcheck[e] : 0;
@@ -376,11 +424,14 @@ int oracle_run(const oracle_problem *p, int nsteps, double *seismo, double *ener
             /* Runge-Kutta (src/rungekuttaMod.f90:27-88) */
             #pragma omp parallel for schedule(static)
             for (size_t i = 0; i < 9 * nglob; ++i)
-                q[i] = wrk == 1 ? rk2(q[i], qn[i], rq[i], dt, irk) : rk3(q[i], qn[i], rq[i], dt, irk, w13, w23);
+                q[i] = wrk == 4 ? lsrk(q[i], &resq[i], rq[i], dt, irk)
+                                : (wrk == 1 ? rk2(q[i], qn[i], rq[i], dt, irk) : rk3(q[i], qn[i], rq[i], dt, irk, w13, w23));
             if (p->attenuation) {
                 #pragma omp parallel for schedule(static)
                 for (size_t i = 0; i < nth; ++i)
-                    th[i] = wrk == 1 ? rk2(th[i], thn[i], arq[i], dt, irk) : rk3(th[i], thn[i], arq[i], dt, irk, w13, w23);
+                    th[i] = wrk == 4 ? lsrk(th[i], &resth[i], arq[i], dt, irk)
+                                     : (wrk == 1 ? rk2(th[i], thn[i], arq[i], dt, irk)
+                                                 : rk3(th[i], thn[i], arq[i], dt, irk, w13, w23));
             }
             if (p->pml) {
                 #pragma omp parallel for schedule(static)
@@ -402,7 +453,11 @@ int oracle_run(const oracle_problem *p, int nsteps, double *seismo, double *ener
                             const double df = -p->alphax[g] * (*f) - p->ddx[g] / p->kx[g] * ((*f) + uf[j * 9 + i]);
                             const double dg = -p->alphay[g] * (*gg) - p->ddy[g] / p->ky[g] * ((*gg) + ug[j * 9 + i]);
                             const double dh = -p->alphaz[g] * (*h) - p->ddz[g] / p->kz[g] * ((*h) + uh[j * 9 + i]);
-                            if (wrk == 1) {
+                            if (wrk == 4) {
+                                *f = lsrk(*f, &resfp[i * nglob + g], df, dt, irk);
+                                *gg = lsrk(*gg, &resfp[(9 + i) * nglob + g], dg, dt, irk);
+                                *h = lsrk(*h, &resfp[(18 + i) * nglob + g], dh, dt, irk);
+                            } else if (wrk == 1) {
                                 *f = rk2(*f, fn, df, dt, irk); *gg = rk2(*gg, gn, dg, dt, irk); *h = rk2(*h, hn, dh, dt, irk);
                             } else {
                                 *f = rk3(*f, fn, df, dt, irk, w13, w23); *gg = rk3(*gg, gn, dg, dt, irk, w13, w23);
@@ -461,9 +516,14 @@ int oracle_run(const oracle_problem *p, int nsteps, double *seismo, double *ener
             }
             energy[it] = en;
         }
+        /* the trigger's switch-off: pmlcheck(:)=.false. after the step
+         * (src/timeloopMod.f90:532-539); the energy stays on */
+        if (p->pml_off_after > 0 && it + 1 == p->pml_off_after)
+            for (int e = 0; e < nelem; ++e) pmlcheck[e] = 0;
     }
     if (q_out) memcpy(q_out, q, sizeof(double) * 9 * nglob);
     free(q); free(qn); free(qm); free(rq); free(ux); free(ax); free(mats); free(pmlcheck);
     free(th); free(thn); free(thm); free(arq); free(fp); free(fpn); free(fpm); free(epml);
+    free(resq); free(resth); free(resfp);
     return 0;
 }

@@ -30,7 +30,7 @@ extern "C" {
 
 typedef struct oracle_problem {
     int nelem, np, npf, nmb;
-    int nt, timeint, attenuation, pml;
+    int nt, timeint, attenuation, pml;   /* timeint 1..3 as the reference; 4: low-storage RK4 */
     int compat;          /* bit 0: RK3 float weights (H3); bit 1: PML uses last element's flux (H1) */
     double dt;
     const double *dr, *ds, *dt_, *lift, *mass;
@@ -48,6 +48,9 @@ typedef struct oracle_problem {
     int nrec;
     const int *recelem;
     const double *rec_int;
+    int pml_off_after;   /* 0: PML stays on; k > 0: the PML elements are switched off after
+                            step k (1-based), as the reference's STA/LTA trigger does
+                            (src/timeloopMod.f90:532-539) */
 } oracle_problem;
 
 /* Runs steps [0, nsteps) of the time loop from the reference's initial
@@ -56,6 +59,12 @@ typedef struct oracle_problem {
  * energy (PML runs) into energy[nt] if non-NULL, and the final q into
  * q_out[9][nglob] if non-NULL.  Returns 0 on success. */
 int oracle_run(const oracle_problem *p, int nsteps, double *seismo, double *energy, double *q_out);
+
+/* The stage arithmetic of timeint 1..4 (Euler, TVD RK2, TVD RK3, the
+ * five-stage low-storage RK4 of Carpenter and Kennedy) applied to the scalar
+ * ODE y' = lam*y for nsteps steps from y0; *y_out receives y.  Used to check
+ * the schemes themselves. */
+int oracle_rk_scalar(int timeint, int compat, double lam, double dt, int nsteps, double y0, double *y_out);
 
 #ifdef __cplusplus
 }

@@ -34,7 +34,7 @@ class OracleProblem(ctypes.Structure):
                [(n, _P) for n in ("ddx", "ddy", "ddz", "alphax", "alphay", "alphaz", "kx", "ky", "kz")] + \
                [("nsrc", ctypes.c_int), ("srcelem", _PI), ("srctype", _PI),
                 ("src_array", _P), ("src_array_m", _P), ("nrec", ctypes.c_int),
-                ("recelem", _PI), ("rec_int", _P)]
+                ("recelem", _PI), ("rec_int", _P), ("pml_off_after", ctypes.c_int)]
 
 
 def lib():
@@ -46,16 +46,29 @@ def lib():
         _LIB = ctypes.CDLL(path)
         _LIB.oracle_run.restype = ctypes.c_int
         _LIB.oracle_run.argtypes = [ctypes.POINTER(OracleProblem), ctypes.c_int, _P, _P, _P]
+        _LIB.oracle_rk_scalar.restype = ctypes.c_int
+        _LIB.oracle_rk_scalar.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                          ctypes.c_double, _P]
     return _LIB
+
+
+def rk_scalar(timeint: int, lam: float, dt: float, nsteps: int, y0: float = 1.0, compat: int = 0) -> float:
+    """The oracle's stage arithmetic of ``timeint`` on y' = lam*y."""
+    y = ctypes.c_double()
+    rc = lib().oracle_rk_scalar(timeint, compat, lam, dt, nsteps, y0, ctypes.byref(y))
+    if rc != 0:
+        raise RuntimeError(f"oracle_rk_scalar failed with {rc}")
+    return y.value
 
 
 def _ptr(a, t=_P):
     return None if a is None else a.ctypes.data_as(t)
 
 
-def run(arrs: dict, nsteps: int | None = None, compat: int = 3):
+def run(arrs: dict, nsteps: int | None = None, compat: int = 3, pml_off_after: int = 0):
     """Run the oracle on reference-layout operands.  Returns
-    (seismograms (nrec,9,nt), energy (nt,), final q (9,nglob))."""
+    (seismograms (nrec,9,nt), energy (nt,), final q (9,nglob)).
+    ``pml_off_after`` = k > 0 switches the PML elements off after step k."""
     nt = arrs["nt"]
     nsteps = nt if nsteps is None else nsteps
     keep = []
@@ -106,6 +119,7 @@ def run(arrs: dict, nsteps: int | None = None, compat: int = 3):
     if p.nrec:
         p.recelem = i(arrs["recelem"])
         p.rec_int = d(arrs["rec_int"])
+    p.pml_off_after = int(pml_off_after)
     seis = np.zeros((max(p.nrec, 1), 9, nt))
     energy = np.zeros(nt)
     q = np.zeros((9, arrs["nelem"] * npn))
@@ -126,9 +140,12 @@ def have_ref_solver(order: int, prec: str = "f64") -> bool:
     return os.access(ref_solver_path(order, prec), os.X_OK)
 
 
-def run_ref_solver(prob, prec: str = "f64", workdir: str | None = None, timeout: int = 3600):
+def run_ref_solver(prob, prec: str = "f64", workdir: str | None = None, timeout: int = 3600,
+                   want_energy: bool = False):
     """Write the one-partition database for ``prob``, run the reference
-    solver on it, and return its seismograms (nrec, 9, nt) plus its stdout."""
+    solver on it, and return its seismograms (nrec, 9, nt) plus its stdout;
+    with ``want_energy`` also the per-step energy it writes to out/energy
+    (runs with PML, src/timeloopMod.f90:652-656), or None without one."""
     import sys
     sys.path.insert(0, os.path.join(os.path.dirname(HERE), "nexd-3d_amd"))
     from nexd_amd import database
@@ -140,4 +157,8 @@ def run_ref_solver(prob, prec: str = "f64", workdir: str | None = None, timeout:
     if r.returncode != 0:
         raise RuntimeError(f"reference solver failed ({r.returncode}): {r.stdout[-2000:]} {r.stderr[-2000:]}")
     seis = database.read_seismograms(os.path.join(tmp, "out"), prob.rec_xyz.shape[0], prob.nt)
-    return seis, r.stdout
+    if not want_energy:
+        return seis, r.stdout
+    fn = os.path.join(tmp, "out", "energy")
+    energy = np.loadtxt(fn, ndmin=2)[:prob.nt, 1] if prob.set_pml and os.path.exists(fn) else None
+    return seis, r.stdout, energy

@@ -3,7 +3,8 @@
 Runs the reference NEXD-3D solver, built from its own sources by
 oracle/build_ref.sh into oracle/_ref/, on one-partition databases written by
 this project's host precompute, and stores the seismograms it writes
-(out/seismo.*) as compressed .npz fixtures next to this script.  Also dumps
+(out/seismo.*) and, for runs with PML, the per-step energy it writes
+(out/energy) as compressed .npz fixtures next to this script.  Also dumps
 the reference-element operators of the reference (oracle/ref_probe.f90) for
 orders 2..5.
 
@@ -41,11 +42,12 @@ def make_case(name: str):
     p = builder(**kw)
     build_ref(p.order)
     t = time.time()
-    seis, out = oracle.run_ref_solver(p)
+    seis, out, energy = oracle.run_ref_solver(p, want_energy=True)
     wall = time.time() - t
     switched = "Energy grows too much" in out
+    extra = {} if energy is None else {"energy": energy}
     np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), seismo=seis, dt=p.dt, nelem=p.nelem,
-                        nt=p.nt, order=p.order, wall_s=wall, pml_switched=switched)
+                        nt=p.nt, order=p.order, wall_s=wall, pml_switched=switched, **extra)
     print(f"{name}: nelem={p.nelem} nt={p.nt} dt={p.dt:.6e} wall={wall:.1f}s switched={switched}", flush=True)
 
 

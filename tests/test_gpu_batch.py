@@ -147,3 +147,34 @@ def test_fp32_batch_other_orders(order, attenuation):
         err = rel_l2(got[k], fp64_single(s))
         print(f"  N={order} att={attenuation} shot {k}: FP32 {err.max():.3e}", flush=True)
         assert err.max() <= 1e-4
+
+
+def test_batch_energy_and_per_shot_pml_switch(layered_shots):
+    """Each shot's energy after every step (src/timeloopMod.f90:508-516)
+    and its own PML switch (src/timeloopMod.f90:532-539): FP64 batch with
+    shot 1 switched off after step k, shot 0 left on, against the oracle
+    run of each shot (seismograms and energy <= 1e-10); the FP32 batch's
+    energies within 1e-4 of FP64."""
+    from oracle import oracle
+    sb = layered_shots
+    nt, k = sb.base.nt, 80
+    with capi.BatchContext(sb, precision=capi.NEXD_FP64, energy=True) as b:
+        b.reset()
+        b.run(1, k)
+        b.set_pml_active(1, False)
+        b.run(k + 1, nt)
+        got = [(b.seismograms(i), b.energy(i)) for i in range(sb.nshot)]
+    for i, s in enumerate(sb.shots):
+        so, eo, _ = oracle.run(layout.reference_arrays(s), compat=3, pml_off_after=k if i == 1 else 0)
+        assert np.max(rel_l2(got[i][0], so)) <= 1e-10
+        assert np.all(eo > 0) and float(np.max(np.abs(got[i][1] - eo) / eo)) <= 1e-10
+    with capi.BatchContext(sb, precision=capi.NEXD_FP32, energy=True) as b:
+        b.reset()
+        b.run(1, k)
+        b.set_pml_active(1, False)
+        b.run(k + 1, nt)
+        e32 = [b.energy(i) for i in range(sb.nshot)]
+    for i in range(sb.nshot):
+        err = float(np.max(np.abs(e32[i] - got[i][1]) / got[i][1]))
+        print(f"  shot {i}: FP32 energy vs FP64 max rel {err:.2e}", flush=True)
+        assert err <= 1e-4

@@ -161,3 +161,108 @@ def test_orders_with_attenuation_and_pml(order, timeint, nt):
     ref, _, _ = oracle.run(arrs, nt)
     err = np.linalg.norm(got - ref, axis=-1) / np.maximum(np.linalg.norm(ref, axis=-1), 1e-300)
     assert err.max() <= 1e-10, err.max()
+
+
+PML_CASES = ["layered4_n3_pml", "layered4_n4_att_pml_rk3", "layered10_n4_att_pml"]
+
+
+@pytest.mark.parametrize("name", PML_CASES)
+def test_gpu_energy_matches_reference_golden(name, golden_dir):
+    """all_energy(it) of every step against what the reference solver wrote
+    to out/energy (src/timeloopMod.f90:508-516,652-656), <= 1e-10 per step."""
+    g = np.load(os.path.join(golden_dir, f"ref_{name}.npz"))
+    p = problem_for(name)
+    print(f"[gpu energy golden] {name} nelem={p.nelem} nt={p.nt}", flush=True)
+    _, _, e = run_gpu(p, energy=True)
+    ref = g["energy"]
+    err = float(np.max(np.abs(e - ref) / ref))
+    print(f"  max rel energy error vs reference = {err:.3e}", flush=True)
+    assert err <= TOL
+
+
+def test_pml_switch_off_matches_oracle():
+    """nexd_set_pml_active(0) after step k, as the reference's trigger does
+    (src/timeloopMod.f90:532-539): seismograms, wavefield and the energy of
+    every step (it stays on after the switch) against the oracle switched
+    off after the same step."""
+    p = problem_for("layered4_n4_att_pml_rk3")
+    arrs = layout.reference_arrays(p)
+    k = 120
+    s_o, e_o, q_o = oracle.run(arrs, compat=3, pml_off_after=k)
+    s_on, _, _ = oracle.run(arrs, compat=3)
+    assert rel_l2(s_o, s_on) > 1e-8          # the switch changes the result
+    with capi.Context(arrs, compat=capi.COMPAT_ALL, energy=True) as ctx:
+        ctx.reset()
+        e1 = ctx.run(1, k, want_energy=True)
+        ctx.set_pml_active(False)
+        e2 = ctx.run(k + 1, p.nt, want_energy=True)
+        s_g, q_g = ctx.seismograms(), ctx.state()
+    e_g = np.concatenate([e1, e2])
+    assert rel_l2(s_g, s_o) <= TOL
+    assert np.max(np.abs(q_g - q_o)) <= TOL * np.max(np.abs(q_o))
+    assert np.all(e_o > 0) and float(np.max(np.abs(e_g - e_o) / e_o)) <= TOL
+
+
+def test_trigger_in_driver_matches_oracle():
+    """The directory driver's STA/LTA loop (nexd_amd.solver) on a run whose
+    threshold makes it fire: it switches the PML off after the step the
+    trigger rule picks from the oracle's own energies, and the run equals
+    the oracle switched off there."""
+    import dataclasses
+    from nexd_amd import solver
+    p0 = problem_for("layered4_n3_pml")
+    p = dataclasses.replace(p0, par=dict(p0.par, avg_window1=10, avg_window2=2, sta_lta_trigger=0.05))
+    arrs = layout.reference_arrays(p)
+    timecrit = solver.trigger_start(p)
+    _, e_full, _ = oracle.run(arrs, compat=3)
+    expect = next((it for it in range(1, p.nt + 1) if solver.sta_lta_fires(e_full, it, p.par, timecrit)), 0)
+    assert timecrit < expect < p.nt
+    with capi.Context(arrs, compat=capi.COMPAT_ALL, energy=True) as ctx:
+        ctx.reset()
+        energy, switched = solver.advance_with_trigger(ctx, p)
+        s_g = ctx.seismograms()
+    print(f"[gpu trigger] timecrit={timecrit} switched after step {switched} (oracle energies: {expect})", flush=True)
+    assert switched == expect
+    s_o, e_o, _ = oracle.run(arrs, compat=3, pml_off_after=switched)
+    assert rel_l2(s_g, s_o) <= TOL
+    assert float(np.max(np.abs(energy - e_o) / e_o)) <= TOL
+
+
+def test_energy_with_pml_run_without_pml_elements():
+    """set_pml with every element outside the PML: the energy still comes
+    from every step (src/timeloopMod.f90:508-516), from the plain list."""
+    p = problem_for("layered4_n3_pml")
+    arrs = dict(layout.reference_arrays(p))
+    arrs["pmlcheck"] = np.zeros_like(arrs["pmlcheck"])
+    s_o, e_o, _ = oracle.run(arrs, compat=3)
+    with capi.Context(arrs, compat=capi.COMPAT_ALL, energy=True) as ctx:
+        ctx.reset()
+        e_g = ctx.run(1, p.nt, want_energy=True)
+        s_g = ctx.seismograms()
+    assert rel_l2(s_g, s_o) <= TOL
+    assert np.all(e_o > 0) and float(np.max(np.abs(e_g - e_o) / e_o)) <= TOL
+
+
+def test_reupload_grows_buffers_and_keeps_receivers_fixed():
+    """A second nexd_upload_sources with larger arrays than the first (a
+    force source after a call that gave none) gets a buffer of the new
+    size; receivers cannot change once steps have run."""
+    p = problem_for("cube3_n2_rk2")
+    arrs = layout.reference_arrays(p)
+    assert arrs["srctype"][0] == 0
+    s_o, _, _ = oracle.run(arrs, compat=3)
+    with capi.Context(arrs) as ctx:
+        L = ctx.lib
+        as_moment = np.ascontiguousarray(np.ones_like(arrs["srctype"]))
+        zeros_m = np.zeros(p.ref.np_ * 6 * arrs["nsrc"] * p.nt)
+        assert L.nexd_upload_sources(ctx.h, capi._i(arrs["srcelem"]), capi._i(as_moment), None,
+                                     capi._d(zeros_m)) == 0
+        assert L.nexd_upload_sources(ctx.h, capi._i(arrs["srcelem"]), capi._i(arrs["srctype"]),
+                                     capi._d(arrs["src_array"]), capi._d(arrs["src_array_m"])) == 0
+        ctx.reset()
+        ctx.run(1, 20)
+        rc = L.nexd_upload_receivers(ctx.h, capi._i(arrs["recelem"]), capi._d(arrs["rec_int"]))
+        assert rc == -3 and b"receivers" in L.nexd_last_error(ctx.h)
+        ctx.run(21, p.nt)
+        s_g = ctx.seismograms()
+    assert rel_l2(s_g, s_o) <= TOL

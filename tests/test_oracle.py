@@ -2,9 +2,12 @@
 
 The golden fixtures (tests/golden/ref_*.npz) are seismograms written by the
 reference NEXD-3D solver (built from its sources into oracle/_ref) on the
-same inputs; the oracle must reproduce them to <= 1e-10 relative L2.  A
-planted mistake in the oracle (dropped compat weight, wrong initial value,
-flipped flux sign) must break that bound.
+same inputs; the oracle must reproduce them to <= 1e-10 relative L2.  For
+runs with PML the fixtures also hold the energy the reference writes after
+every step (out/energy), which the oracle must reproduce to <= 1e-12 per
+step.  A planted mistake in the oracle (dropped compat weight, wrong initial
+value, flipped flux sign, swapped Lame parameters in the energy) must break
+those bounds.
 """
 import os
 
@@ -72,3 +75,112 @@ def test_oracle_matches_live_reference_solver():
     sr, out = oracle.run_ref_solver(p)
     s, _, _ = oracle.run(layout.reference_arrays(p), compat=3)
     assert rel_l2(s, sr) <= TOL
+
+
+ENERGY_TOL = 1e-12
+
+
+@pytest.mark.parametrize("name", ["layered4_n3_pml", "layered4_n4_att_pml_rk3"])
+def test_oracle_energy_matches_reference_golden(name, golden_dir):
+    p, g = _case(name, golden_dir)
+    assert "energy" in g.files
+    _, e, _ = oracle.run(layout.reference_arrays(p), compat=3)
+    ref = g["energy"]
+    assert ref.shape == e.shape and np.all(ref > 0)
+    assert float(np.max(np.abs(e - ref) / ref)) <= ENERGY_TOL
+
+
+def test_energy_planted_mistake_is_caught(golden_dir):
+    """mu and lambda enter only the energy's strain (src/timeloopMod.f90:
+    490-500): swapping them leaves the seismograms alone and must break the
+    energy bound."""
+    p, g = _case("layered4_n3_pml", golden_dir)
+    arrs = dict(layout.reference_arrays(p))
+    arrs["mu"], arrs["lambda"] = arrs["lambda"], arrs["mu"]
+    s, e, _ = oracle.run(arrs, compat=3)
+    assert rel_l2(s, g["seismo"]) <= TOL
+    assert float(np.max(np.abs(e - g["energy"]) / g["energy"])) > 1e-3
+
+
+def test_sta_lta_trigger_rule():
+    """The trigger of src/timeloopMod.f90:517-540 on a hand-made energy
+    series: windows of 4 (long) and 2 (short) energies before step it,
+    switch when |short/long - 1| > threshold, never at it <= timecrit."""
+    from nexd_amd import solver
+    par = dict(avg_window1=4, avg_window2=2, sta_lta_trigger=0.1)
+    energy = np.array([1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.3, 1.3, 5.0, 5.0])
+    # after step 7: long = steps 3..6 = 1, short = steps 5, 6 = 1 -> ratio 1, no switch
+    assert not solver.sta_lta_fires(energy, 7, par, timecrit=2)
+    # after step 9: long = (1 + 1 + 1.3 + 1.3) / 4 = 1.15, short = 1.3 -> |1.130 - 1| > 0.1
+    assert solver.sta_lta_fires(energy, 9, par, timecrit=2)
+    assert not solver.sta_lta_fires(energy, 9, dict(par, sta_lta_trigger=0.14), timecrit=2)
+    assert not solver.sta_lta_fires(energy, 9, par, timecrit=9)
+    # only steps >= 1 enter (it - i > 0), each divided by the full window:
+    # after step 2, long = 1/4 and short = 1/2 -> ratio 2
+    assert solver.sta_lta_fires(energy, 2, par, timecrit=0)
+
+
+def _amplification_coefficients(timeint, degree, compat=0):
+    """Coefficients of R(z) = y(dt)/y(0) for y' = z*y over one step, fitted
+    exactly through degree+1 sample points z."""
+    z = np.linspace(-1.5, 0.5, degree + 1)
+    r = np.array([oracle.rk_scalar(timeint, float(zi), 1.0, 1, compat=compat) for zi in z])
+    return np.linalg.solve(np.vander(z, increasing=True), r)
+
+
+def test_rk_schemes_amplification_polynomials():
+    """The stage arithmetic of every scheme, on the scalar ODE: an s-stage
+    explicit scheme of order p has R(z) = sum_{k<=p} z^k/k! + O(z^{p+1}),
+    a polynomial of degree s.  TVD RK2 (2 stages, order 2) and TVD RK3
+    (3, 3) are exactly their Taylor polynomials; the low-storage RK4 (5
+    stages, order 4) matches 1, 1, 1/2, 1/6, 1/24 and has a non-zero z^5
+    term; with the reference's float-rounded RK3 weights the z^3 term is
+    off by the rounding."""
+    taylor = np.array([1.0, 1.0, 0.5, 1.0 / 6.0, 1.0 / 24.0])
+    c = _amplification_coefficients(2, 4)
+    assert np.allclose(c, [1.0, 1.0, 0.5, 0.0, 0.0], atol=1e-12)
+    c = _amplification_coefficients(3, 5)
+    assert np.allclose(c, [1.0, 1.0, 0.5, 1.0 / 6.0, 0.0, 0.0], atol=1e-12)
+    c = _amplification_coefficients(3, 3, compat=1)
+    assert abs(c[3] - 1.0 / 6.0) > 1e-9
+    c = _amplification_coefficients(4, 6)
+    assert np.allclose(c[:5], taylor, atol=1e-11)
+    assert abs(c[5]) > 1e-4 and abs(c[6]) < 1e-10
+
+
+def test_lsrk4_converges_at_fourth_order():
+    """Global error at t = 2 of y' = lam*y (lam = -1, -3) falls about
+    16-fold per halving of dt once dt is in the asymptotic range."""
+    for lam in (-1.0, -3.0):
+        errs = []
+        for n in (20, 40, 80):
+            y = oracle.rk_scalar(4, lam, 2.0 / n, n)
+            errs.append(abs(y - np.exp(lam * 2.0)))
+        ratios = [errs[0] / errs[1], errs[1] / errs[2]]
+        assert all(14.0 < r < 18.0 for r in ratios), ratios
+
+
+def test_lsrk4_checks_catch_a_wrong_coefficient():
+    """The two checks above detect a plausible slip in the coefficient table:
+    the same five stages restated with one B coefficient perturbed in its
+    fourth digit break the order conditions."""
+    a = [0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+         -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0]
+    b = [1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+         1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+         2277821191437.0 / 14882151754819.0]
+
+    def step(z, bb):
+        y, res = 1.0, 0.0
+        for k in range(5):
+            res = a[k] * res + z * y
+            y = y + bb[k] * res
+        return y
+    zs = np.linspace(-1.5, 0.5, 7)
+    good = np.linalg.solve(np.vander(zs, increasing=True), [step(z, b) for z in zs])
+    assert np.allclose(good[:5], [1.0, 1.0, 0.5, 1.0 / 6.0, 1.0 / 24.0], atol=1e-11)
+    assert np.allclose(good, _amplification_coefficients(4, 6), atol=1e-12)
+    bad_b = list(b)
+    bad_b[2] *= 1.0001
+    bad = np.linalg.solve(np.vander(zs, increasing=True), [step(z, bad_b) for z in zs])
+    assert not np.allclose(bad[:5], [1.0, 1.0, 0.5, 1.0 / 6.0, 1.0 / 24.0], atol=1e-6)
