@@ -235,6 +235,8 @@ int nexd_batch_run(nexd_batch *b, int it_begin, int it_end);
 /* seismograms of one shot, each (nrec,nt) in double */
 int nexd_batch_get_seismograms(nexd_batch *b, int shot, double *ux, double *uy, double *uz,
                                double *vx, double *vy, double *vz, double *ax, double *ay, double *az);
+/* wavefield q (nglob,9) of one shot after the steps run so far, in double */
+int nexd_batch_get_state(nexd_batch *b, int shot, double *q);
 /* energy of one shot after every step recorded so far (nt values; config
  * energy and pml), as nexd_run returns it for one context */
 int nexd_batch_get_energy(nexd_batch *b, int shot, double *energy);

@@ -691,6 +691,17 @@ int get_seis(Batch<R> *b, int shot, double *const outs[9]) {
 }
 
 template <typename R>
+int get_state(Batch<R> *b, int shot, double *q) {
+    if (shot < 0 || shot >= b->cfg.nshot || !q) return b->fail(NEXD_E_ARG, "shot out of range or null output");
+    const size_t n = 9 * b->ng;
+    std::vector<R> h(n);
+    hipError_t e = hipMemcpy(h.data(), b->q[0] + (size_t)shot * n, sizeof(R) * n, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return b->fail(NEXD_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    for (size_t i = 0; i < n; ++i) q[i] = (double)h[i];
+    return 0;
+}
+
+template <typename R>
 int get_energy(Batch<R> *b, int shot, double *out) {
     if (shot < 0 || shot >= b->cfg.nshot || !out) return b->fail(NEXD_E_ARG, "shot out of range or null output");
     if (!b->energy_on()) return b->fail(NEXD_E_STATE, "batch created without energy or without PML");
@@ -930,6 +941,11 @@ int nexd_batch_get_seismograms(nexd_batch *b, int shot, double *ux, double *uy, 
     if (!b) return NEXD_E_ARG;
     double *const outs[9] = {ux, uy, uz, vx, vy, vz, ax, ay, az};
     return base(b)->cfg.precision == NEXD_FP32 ? get_seis(as<float>(b), shot, outs) : get_seis(as<double>(b), shot, outs);
+}
+
+int nexd_batch_get_state(nexd_batch *b, int shot, double *q) {
+    if (!b) return NEXD_E_ARG;
+    return base(b)->cfg.precision == NEXD_FP32 ? get_state(as<float>(b), shot, q) : get_state(as<double>(b), shot, q);
 }
 
 int nexd_batch_get_energy(nexd_batch *b, int shot, double *energy) {

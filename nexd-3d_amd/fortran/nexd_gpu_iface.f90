@@ -151,4 +151,16 @@ contains
         end do
     end function nexd_last_error
 
+    ! stops the program with the library's message when rc is an error
+    ! (the reference stops on errors too, src/mpiMod.f90:287-307)
+    subroutine nexd_check(ctx, rc, what)
+        type(c_ptr), value :: ctx
+        integer(c_int), intent(in) :: rc
+        character(len=*), intent(in) :: what
+        if (rc /= NEXD_OK) then
+            write(*,*) 'libnexd_gpu: ', what, ' failed (', rc, '): ', trim(nexd_last_error(ctx))
+            error stop 1
+        end if
+    end subroutine nexd_check
+
 end module nexd_gpu_iface

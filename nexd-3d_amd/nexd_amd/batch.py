@@ -131,7 +131,7 @@ def layered_shots(nshot: int, seed: int = 20, full_source_arrays: bool = False, 
 
 def uneven_instances(count: int, seed: int = 4000, n_lo: int = 10, n_hi: int = 30, orders=(2, 3, 4, 5),
                      nt: int = 200, hsize: float = 500.0, nrec: int = 8, f0: float = 1.0,
-                     timeint: int = 2) -> list:
+                     timeint: int = 2, full_source_arrays: bool = False, log=None) -> list:
     """Independent instances of different size and order (one shot each):
     n uniform in [n_lo, n_hi] hexes per side (6 n^3 tets), N uniform in
     ``orders``, 2-4 horizontal elastic layers with vp uniform in 2500-6500
@@ -161,6 +161,8 @@ def uneven_instances(count: int, seed: int = 4000, n_lo: int = 10, n_hi: int = 3
         src = srcmod.Source(xyz=xyz, kind=0, stf=2, f0=f0, factor=1e10, angle_force=(0.0, 0.0, 1.0))
         recs = [((i + 0.5) * ext / nrec, ext / 2, ext) for i in range(nrec)]
         p = problem.setup(m, order, dict(timeint=timeint, nt=nt, cfl=0.25), [src], recs,
-                          name=f"uneven{k}_n{n}_N{order}", full_source_arrays=False)
+                          name=f"uneven{k}_n{n}_N{order}", full_source_arrays=full_source_arrays)
         out.append(ShotBatch(p, [p], True))
+        if log:
+            log(f"instance {k}: {p.nelem} tets, N={order}")
     return out

@@ -201,7 +201,7 @@ NEXD_FP64, NEXD_FP32 = 0, 1
 BATCH_EXPORTS = ["nexd_batch_create", "nexd_batch_upload_operators", "nexd_batch_upload_mesh",
                  "nexd_batch_upload_material", "nexd_batch_upload_anelastic", "nexd_batch_upload_pml",
                  "nexd_batch_upload_sources", "nexd_batch_upload_receivers", "nexd_batch_reset", "nexd_batch_run",
-                 "nexd_batch_get_seismograms", "nexd_batch_get_energy", "nexd_batch_set_pml_active",
+                 "nexd_batch_get_seismograms", "nexd_batch_get_state", "nexd_batch_get_energy", "nexd_batch_set_pml_active",
                  "nexd_batch_last_run_ms", "nexd_batch_time_stage_kernel",
                  "nexd_batch_destroy", "nexd_batch_last_error"]
 GROUP_EXPORTS = ["nexd_group_create", "nexd_group_run", "nexd_group_last_run_ms", "nexd_group_destroy",
@@ -239,6 +239,7 @@ class BatchContext:
         base = sb.base
         a = layout.reference_arrays(base)
         self.nshot, self.nt, self.nrec = sb.nshot, base.nt, a["nrec"]
+        self.nglob = a["nelem"] * a["np"]
         cfg = NexdBatchConfig(nelem=a["nelem"], order=a["order"], nshot=sb.nshot, nsrc=len(sb.shots[0].sources),
                               nrec=a["nrec"], nt=a["nt"], timeint=a["timeint"], attenuation=a["attenuation"],
                               pml=a["pml"], compat=compat, precision=precision,
@@ -291,6 +292,12 @@ class BatchContext:
         outs = [np.zeros(self.nrec * self.nt) for _ in range(9)]
         self._chk(self.lib.nexd_batch_get_seismograms(self.h, shot, *[_d(o) for o in outs]), "get_seismograms")
         return np.stack([o.reshape(self.nt, self.nrec).T for o in outs], axis=1)
+
+    def state(self, shot: int) -> np.ndarray:
+        """q of one shot as (9, nglob)."""
+        out = np.zeros(9 * self.nglob)
+        self._chk(self.lib.nexd_batch_get_state(self.h, shot, _d(out)), "get_state")
+        return out.reshape(9, self.nglob)
 
     def energy(self, shot: int) -> np.ndarray:
         """all_energy of every step of one shot (nt,)."""

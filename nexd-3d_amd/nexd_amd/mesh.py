@@ -350,16 +350,25 @@ def build_geometry(m: TetMesh, order: int, vpu: np.ndarray | None = None) -> Geo
     cand = nb[:, None] * npn + ref.fmask.T[nbf]               # (nint, NpF)
     own = ibt[e_idx, f_idx]                                   # (nint, NpF)
     gx, gy, gz = vx.ravel(), vy.ravel(), vz.ravel()
-    # coordinate matching of face nodes (src/meshMod.f90:955-987)
-    chunk = 200000
-    perm_nodes = np.empty_like(cand)
-    for s0 in range(0, cand.shape[0], chunk):
-        sl = slice(s0, s0 + chunk)
-        o, cc = own[sl], cand[sl]
+    # coordinate matching of face nodes (src/meshMod.f90:955-987): each own
+    # face node takes the neighbour face node nearest to it.  Both node sets
+    # are sorted along one generic direction, which pairs coincident nodes;
+    # pairs that do not coincide (near-ties of the sort key) are matched by
+    # the full nearest-node search instead.
+    key = np.array([1.0, 0.5 ** 0.5, 0.3 ** 0.5])
+    kg = gx * key[0] + gy * key[1] + gz * key[2]
+    so = np.take_along_axis(own, np.argsort(kg[own], axis=1), axis=1)
+    sc = np.take_along_axis(cand, np.argsort(kg[cand], axis=1), axis=1)
+    dist = np.abs(gx[so] - gx[sc]) + np.abs(gy[so] - gy[sc]) + np.abs(gz[so] - gz[sc])
+    scale = np.abs(gx[so]) + np.abs(gy[so]) + np.abs(gz[so]) + 1.0
+    # the candidate paired with own[f, i]: sorted position of own[f, i] in sc
+    perm_nodes = np.take_along_axis(sc, np.argsort(np.argsort(kg[own], axis=1), axis=1), axis=1)
+    bad = np.nonzero(np.any(dist > 1e-9 * scale, axis=1))[0]
+    if bad.size:
+        o, cc = own[bad], cand[bad]
         d = (np.abs(gx[o][:, :, None] - gx[cc][:, None, :]) + np.abs(gy[o][:, :, None] - gy[cc][:, None, :]) +
              np.abs(gz[o][:, :, None] - gz[cc][:, None, :]))
-        k = np.argmin(d, axis=2)
-        perm_nodes[sl] = np.take_along_axis(cc, k, axis=1)
+        perm_nodes[bad] = np.take_along_axis(cc, np.argmin(d, axis=2), axis=1)
     ibn[e_idx, f_idx] = perm_nodes
 
     if vpu is None:

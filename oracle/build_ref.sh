@@ -20,13 +20,22 @@
 # solver path, and the solver executable is linked with
 # --allow-shlib-undefined.  The mesher is not built.
 #
-# usage: oracle/build_ref.sh <order> <f64|f32>
+# With a third argument "gpu" the build is the reference solver whose time
+# loop calls libnexd_gpu.so: the fragments of
+# nexd-3d_amd/fortran/timeloop3d_gpu_patch.f90 are spliced into the scratch
+# copy of src/timeloopMod.f90 (oracle/splice_timeloop.py), the modules are
+# compiled with nexd_gpu_iface, and the solver links the library, into
+# oracle/_ref/o<N>_f64_gpu/.
+#
+# usage: oracle/build_ref.sh <order> <f64|f32> [gpu]
 set -euo pipefail
 ORDER=${1:?order}
 PREC=${2:-f64}
+VARIANT=${3:-}
 REF=${NEXD_REFERENCE:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)
-OUT=$HERE/_ref/o${ORDER}_${PREC}
+OUT=$HERE/_ref/o${ORDER}_${PREC}${VARIANT:+_$VARIANT}
+PKG=$HERE/../nexd-3d_amd
 SCRATCH=$(mktemp -d /tmp/nexdref_o${ORDER}_${PREC}.XXXXXX)
 trap 'rm -rf "$SCRATCH"' EXIT
 mkdir -p "$OUT"
@@ -63,24 +72,37 @@ awk '
 
 FC=${FC:-amdflang}
 FFLAGS="-O2 -fPIC -ffixed-line-length=none -I$SCRATCH -I/opt/conda/include -module-dir $SCRATCH"
+GPUOBJ=""
+GPULIB=""
+if [ "$VARIANT" = gpu ]; then
+  [ "$PREC" = f64 ] || { echo "the library takes REAL(8): gpu builds are f64" >&2; exit 2; }
+  [ -f "$PKG/libnexd_gpu.so" ] || { echo "build libnexd_gpu.so first (make -C nexd-3d_amd)" >&2; exit 2; }
+  python3 "$HERE/splice_timeloop.py" "$REF/src/timeloopMod.f90" "$PKG/fortran/timeloop3d_gpu_patch.f90" \
+      "$SCRATCH/timeloopMod.f90"
+  $FC $FFLAGS -c "$PKG/fortran/nexd_gpu_iface.f90" -o "$SCRATCH/nexd_gpu_iface.o"
+  GPUOBJ="$SCRATCH/nexd_gpu_iface.o"
+  GPULIB="-L$PKG -lnexd_gpu -Wl,-rpath,\$ORIGIN/../../../nexd-3d_amd"
+fi
 MODS="mpiincludeMod typeMod fileParameterMod parameterMod logoMod allocateMod mpiMod
       rosettaGammaMod jacobiMod matrixMod linearSystemMod tetTrafoMod simplexMod
       vandermondeMod gllMod warpfactorMod nodesMod dmatricesMod liftMod
       geometricFactorsMod normalsMod stfMod sourceReceiverMod anelasticMod waveMod
       tetraNeighborMod vtkMod plotMod pmlMod rungekuttaMod meshMod timeloopMod
       collectMovieMod"
-OBJS="$SCRATCH/constantsMod.o"
+OBJS="$SCRATCH/constantsMod.o $GPUOBJ"
 $FC $FFLAGS -c "$SCRATCH/constantsMod.f90" -o "$SCRATCH/constantsMod.o"
 for m in $MODS; do
   srcf="$REF/src/$m.f90"
   [ "$m" = meshMod ] && srcf="$SCRATCH/meshMod.f90"
+  [ "$m" = timeloopMod ] && [ "$VARIANT" = gpu ] && srcf="$SCRATCH/timeloopMod.f90"
   $FC $FFLAGS -c "$srcf" -o "$SCRATCH/$m.o" 2> "$SCRATCH/$m.log" || { cat "$SCRATCH/$m.log"; exit 1; }
   OBJS="$OBJS $SCRATCH/$m.o"
 done
 LIBS="-L/opt/conda/lib -Wl,-rpath,/opt/conda/lib -lmpifort -lmpi -lmkl_rt"
-$FC -shared -o "$OUT/libnexdref.so" $OBJS $LIBS
+$FC -shared -o "$OUT/libnexdref.so" $OBJS $LIBS $GPULIB
 $FC $FFLAGS -c "$REF/src/program_solver.f90" -o "$SCRATCH/program_solver.o"
-$FC -o "$OUT/solver" "$SCRATCH/program_solver.o" -L"$OUT" -Wl,-rpath,'$ORIGIN' -lnexdref $LIBS -Wl,--allow-shlib-undefined
+$FC -o "$OUT/solver" "$SCRATCH/program_solver.o" -L"$OUT" -Wl,-rpath,'$ORIGIN' -lnexdref $LIBS $GPULIB \
+    -Wl,--allow-shlib-undefined
 $FC $FFLAGS -c "$HERE/ref_probe.f90" -o "$SCRATCH/ref_probe.o"
 $FC -o "$OUT/ref_probe" "$SCRATCH/ref_probe.o" -L"$OUT" -Wl,-rpath,'$ORIGIN' -lnexdref $LIBS -Wl,--allow-shlib-undefined
 echo "built $OUT"

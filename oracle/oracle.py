@@ -132,27 +132,29 @@ def run(arrs: dict, nsteps: int | None = None, compat: int = 3, pml_off_after: i
 # ---------------------------------------------------------------------------
 # The reference solver built from its own sources (oracle/build_ref.sh)
 # ---------------------------------------------------------------------------
-def ref_solver_path(order: int, prec: str = "f64") -> str:
-    return os.path.join(HERE, "_ref", f"o{order}_{prec}", "solver")
+def ref_solver_path(order: int, prec: str = "f64", variant: str = "") -> str:
+    return os.path.join(HERE, "_ref", f"o{order}_{prec}" + (f"_{variant}" if variant else ""), "solver")
 
 
-def have_ref_solver(order: int, prec: str = "f64") -> bool:
-    return os.access(ref_solver_path(order, prec), os.X_OK)
+def have_ref_solver(order: int, prec: str = "f64", variant: str = "") -> bool:
+    return os.access(ref_solver_path(order, prec, variant), os.X_OK)
 
 
 def run_ref_solver(prob, prec: str = "f64", workdir: str | None = None, timeout: int = 3600,
-                   want_energy: bool = False):
+                   want_energy: bool = False, variant: str = ""):
     """Write the one-partition database for ``prob``, run the reference
     solver on it, and return its seismograms (nrec, 9, nt) plus its stdout;
     with ``want_energy`` also the per-step energy it writes to out/energy
-    (runs with PML, src/timeloopMod.f90:652-656), or None without one."""
+    (runs with PML, src/timeloopMod.f90:652-656), or None without one.
+    ``variant`` "gpu" runs the build whose time loop is libnexd_gpu.so
+    (oracle/build_ref.sh <order> f64 gpu)."""
     import sys
     sys.path.insert(0, os.path.join(os.path.dirname(HERE), "nexd-3d_amd"))
     from nexd_amd import database
     real = "<f8" if prec == "f64" else "<f4"
     tmp = workdir or tempfile.mkdtemp(prefix="nexdref_")
     database.write_run_directory(tmp, prob, real)
-    r = subprocess.run([ref_solver_path(prob.order, prec)], cwd=tmp, capture_output=True,
+    r = subprocess.run([ref_solver_path(prob.order, prec, variant)], cwd=tmp, capture_output=True,
                        text=True, timeout=timeout)
     if r.returncode != 0:
         raise RuntimeError(f"reference solver failed ({r.returncode}): {r.stdout[-2000:]} {r.stderr[-2000:]}")
