@@ -130,6 +130,19 @@ int nexd_reset(nexd_ctx *ctx);
  * context computes energy, energy[k] receives all_energy(it_begin+k). */
 int nexd_run(nexd_ctx *ctx, int it_begin, int it_end, double *energy);
 
+/* No counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  nexd_run without any allocation: the step kernels run
+ * directly on the context stream (no graph is instantiated) and keep their
+ * scratch -- the energy block partials and the last element's fluxes --
+ * in a static per-GPU array of 2^20 doubles (8 MiB) instead of the
+ * context's buffers.  Needs a context that was reset (nexd_reset or
+ * nexd_set_state; NEXD_E_STATE otherwise); contexts whose scratch exceeds
+ * the array (about 4 million elements) get NEXD_E_UNSUPPORTED.  All calls on
+ * one GPU share the array: calls that overlap in time on one GPU race.
+ * Same arguments and results as nexd_run. */
+int nexd_run_noalloc(nexd_ctx *ctx, int it_begin, int it_end, double *energy);
+
 /* PML switch used by the caller's STA/LTA trigger (src/timeloopMod.f90:532-539). */
 int nexd_set_pml_active(nexd_ctx *ctx, int active);
 

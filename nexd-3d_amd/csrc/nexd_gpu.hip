@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -151,20 +152,28 @@ void dispatch_stage(int order, const StageArgs<double> &a, bool att, bool pml, h
     else nexd::launch_stage<double, false>(order, att, pml, a, s);
 }
 
+// Where a step's kernels keep their scratch: the energy block partials
+// (nblk_total values) and the last element's 27 fluxes (27*Np values).
+// nexd_run uses the context's buffers, nexd_run_noalloc a static array.
+struct StepScratch {
+    double *blk_energy;
+    double *last_flux;
+};
+
 template <int ORDER>
-void launch_last_flux(nexd_ctx *c, const double *qin, const double *pin) {
+void launch_last_flux(nexd_ctx *c, const double *qin, const double *pin, double *out) {
     const int last = c->cfg.nelem - 1;
     hipLaunchKernelGGL((nexd::last_flux_kernel<ORDER, double>), dim3(1), dim3(64), 0, c->stream, qin, pin, c->pmlc,
-                       c->matc, last, c->pmlcheck[last] ? 1 : 0, (int)c->nglob, 0, 0, c->last_flux);
+                       c->matc, last, c->pmlcheck[last] ? 1 : 0, (int)c->nglob, 0, 0, out);
 }
 
-void dispatch_last_flux(nexd_ctx *c, const double *qin, const double *pin) {
+void dispatch_last_flux(nexd_ctx *c, const double *qin, const double *pin, double *out) {
     switch (c->cfg.order) {
-        case 1: launch_last_flux<1>(c, qin, pin); break;
-        case 2: launch_last_flux<2>(c, qin, pin); break;
-        case 3: launch_last_flux<3>(c, qin, pin); break;
-        case 4: launch_last_flux<4>(c, qin, pin); break;
-        default: launch_last_flux<5>(c, qin, pin); break;
+        case 1: launch_last_flux<1>(c, qin, pin, out); break;
+        case 2: launch_last_flux<2>(c, qin, pin, out); break;
+        case 3: launch_last_flux<3>(c, qin, pin, out); break;
+        case 4: launch_last_flux<4>(c, qin, pin, out); break;
+        default: launch_last_flux<5>(c, qin, pin, out); break;
     }
 }
 
@@ -174,22 +183,23 @@ void dispatch_last_flux(nexd_ctx *c, const double *qin, const double *pin) {
 bool energy_on(const nexd_ctx *c) { return c->cfg.energy && c->cfg.pml && c->have_pml; }
 
 template <int ORDER>
-void launch_step_end(nexd_ctx *c, int row_offset, int advance) {
-    const double *be = energy_on(c) ? c->blk_energy : nullptr;
+void launch_step_end(nexd_ctx *c, int row_offset, int advance, const double *blk_energy) {
+    const double *be = energy_on(c) ? blk_energy : nullptr;
     hipLaunchKernelGGL((nexd::step_end_kernel<ORDER, double>), dim3(1), dim3(1024), 0, c->stream, c->q[0], c->u_slot,
                        c->a_slot, c->rec_int, c->recelem, c->rec_slot_of, c->cfg.nrec, c->nslot, 1, (int)c->nglob,
                        c->cfg.nt, row_offset, advance, c->seis, be, c->nblk_total, c->energy, c->d_it);
 }
 
-void dispatch_step_end(nexd_ctx *c, int row_offset, int advance) {
+void dispatch_step_end(nexd_ctx *c, int row_offset, int advance, const double *blk_energy) {
     switch (c->cfg.order) {
-        case 1: launch_step_end<1>(c, row_offset, advance); break;
-        case 2: launch_step_end<2>(c, row_offset, advance); break;
-        case 3: launch_step_end<3>(c, row_offset, advance); break;
-        case 4: launch_step_end<4>(c, row_offset, advance); break;
-        default: launch_step_end<5>(c, row_offset, advance); break;
+        case 1: launch_step_end<1>(c, row_offset, advance, blk_energy); break;
+        case 2: launch_step_end<2>(c, row_offset, advance, blk_energy); break;
+        case 3: launch_step_end<3>(c, row_offset, advance, blk_energy); break;
+        case 4: launch_step_end<4>(c, row_offset, advance, blk_energy); break;
+        default: launch_step_end<5>(c, row_offset, advance, blk_energy); break;
     }
 }
+void dispatch_step_end(nexd_ctx *c, int row_offset, int advance) { dispatch_step_end(c, row_offset, advance, c->blk_energy); }
 
 StageArgs<double> base_args(nexd_ctx *c) {
     const bool f32w = (c->cfg.compat & NEXD_COMPAT_RK3_FLOAT_WEIGHTS) != 0;
@@ -257,11 +267,14 @@ void stage_buffers(nexd_ctx *c, int irk, StageArgs<double> &a) {
     a.lsb = ls ? nexd::kLsrkB[irk - 1] : 0.0;
 }
 
-void enqueue_step(nexd_ctx *c) {
+// one full time step on the context stream; `fork_ok` lets the interior list
+// run on the side stream beside the PML list (graph capture)
+void enqueue_step(nexd_ctx *c, const StepScratch &sc, bool fork_ok) {
     const int scheme = c->cfg.timeint;
     const bool att = c->cfg.attenuation != 0;
     const bool pml = c->have_pml && c->pml_active && c->n_pml > 0;
     StageArgs<double> a = base_args(c);
+    a.last_flux = sc.last_flux;
     const int nrk = c->nrk;
     for (int irk = 1; irk <= nrk; ++irk) {
         stage_buffers(c, irk, a);
@@ -272,28 +285,28 @@ void enqueue_step(nexd_ctx *c) {
         // elements without PML (every element when PML is off); with PML they
         // run on a side stream, beside the PML list: both read the stage
         // input and write disjoint elements and energy partials
-        const bool fork = pml && c->n_plain > 0;
+        const bool fork = fork_ok && pml && c->n_plain > 0;
         if (fork) {
             (void)hipEventRecord(c->ev_fork, c->stream);
             (void)hipStreamWaitEvent(c->side, c->ev_fork, 0);
         }
         a.elist = c->elist_plain;
         a.nlist = c->n_plain;
-        a.blk_energy = energy ? c->blk_energy : nullptr;
+        a.blk_energy = energy ? sc.blk_energy : nullptr;
         dispatch_stage(c->cfg.order, a, att, false, fork ? c->side : c->stream);
         if (fork) (void)hipEventRecord(c->ev_join, c->side);
         if (pml) {
-            if (a.pml_compat) dispatch_last_flux(c, qin, pin);
+            if (a.pml_compat) dispatch_last_flux(c, qin, pin, sc.last_flux);
             a.elist = c->elist_pml;
             a.nlist = c->n_pml;
-            a.blk_energy = energy ? c->blk_energy + nblocks(c->cfg.order, c->n_plain, att, false) : nullptr;
+            a.blk_energy = energy ? sc.blk_energy + nblocks(c->cfg.order, c->n_plain, att, false) : nullptr;
             dispatch_stage(c->cfg.order, a, att, true, c->stream);
         }
         if (fork) (void)hipStreamWaitEvent(c->stream, c->ev_join, 0);
         if (scheme == NEXD_EULER)
             (void)hipMemcpyAsync(c->q[0], c->q[1], sizeof(double) * 9 * c->nglob, hipMemcpyDeviceToDevice, c->stream);
     }
-    dispatch_step_end(c, 1, 1);
+    dispatch_step_end(c, 1, 1, sc.blk_energy);
 }
 
 int rebuild_lists(nexd_ctx *c) {
@@ -319,6 +332,37 @@ int rebuild_lists(nexd_ctx *c) {
     const bool att = c->cfg.attenuation != 0;
     c->nblk_total = nblocks(c->cfg.order, c->n_plain, att, false) + nblocks(c->cfg.order, c->n_pml, att, true);
     drop_graph(c);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of nexd_run_noalloc: one static array per GPU, resolved once per
+// device to its device address (hipGetSymbolAddress) at context creation.
+// Every nexd_run_noalloc on a GPU shares that GPU's array.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchReals = (size_t)1 << 20;   // 8 MiB of doubles per GPU
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+
+}  // namespace
+
+__device__ __align__(256) static double scratch[kScratchReals];
+
+namespace {
+
+// the device address of `scratch` on the current device (called with that
+// device current); 0 on success
+int resolve_scratch(nexd_ctx *c) {
+    const int dev = c->cfg.device;
+    if (dev >= kMaxDevices) return fail(c, NEXD_E_UNSUPPORTED, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[dev]) return 0;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(c, NEXD_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[dev] = static_cast<double *>(p);
     return 0;
 }
 
@@ -356,6 +400,10 @@ int nexd_create(const nexd_config *cfg, nexd_ctx **out) {
         hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
         return bail(fail(c, NEXD_E_HIP, "hipEventCreate failed"));
+    {
+        const int rs = resolve_scratch(c);
+        if (rs) return bail(rs);
+    }
     const size_t ng = c->nglob;
     int rc = 0;
     for (int i = 0; i < 3 && !rc; ++i) rc = dalloc(c, &c->q[i], 9 * ng);
@@ -593,7 +641,7 @@ int nexd_run(nexd_ctx *c, int it_begin, int it_end, double *energy) {
     if (!c->graph) {
         hipGraph_t g = nullptr;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_step(c);
+        enqueue_step(c, StepScratch{c->blk_energy, c->last_flux}, true);
         HIPCHK(c, hipStreamEndCapture(c->stream, &g));
         hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
@@ -601,6 +649,36 @@ int nexd_run(nexd_ctx *c, int it_begin, int it_end, double *energy) {
     }
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     for (int it = it_begin; it <= it_end; ++it) HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+    if (energy && energy_on(c))
+        HIPCHK(c, hipMemcpy(energy, c->energy + (it_begin - 1), sizeof(double) * (it_end - it_begin + 1), hipMemcpyDeviceToHost));
+    c->step = it_end;
+    return NEXD_OK;
+}
+
+int nexd_run_noalloc(nexd_ctx *c, int it_begin, int it_end, double *energy) {
+    if (!c) return fail(c, NEXD_E_ARG, "null context");
+    if (!c->have_ops || !c->have_mesh || !c->have_mat || !c->have_att || (c->cfg.pml && !c->have_pml) ||
+        !c->have_src || !c->have_rec)
+        return fail(c, NEXD_E_STATE, "upload operators, mesh, material, anelastic/PML, sources and receivers first");
+    if (!c->initialized) return fail(c, NEXD_E_STATE, "call nexd_reset (or nexd_set_state) before nexd_run_noalloc");
+    if (it_begin != c->step + 1) return fail(c, NEXD_E_ARG, "it_begin must be the next step");
+    if (it_end < it_begin || it_end > c->cfg.nt) return fail(c, NEXD_E_ARG, "it_end out of range");
+    // scratch: the energy block partials, then the last element's fluxes
+    const size_t need = (size_t)c->nblk_total + 27 * (size_t)c->np;
+    if (need > kScratchReals)
+        return fail(c, NEXD_E_UNSUPPORTED, "nexd_run_noalloc: the mesh needs more scratch than the static 2^20 values "
+                                           "(use nexd_run)");
+    double *base = g_scratch[c->cfg.device];
+    if (!base) return fail(c, NEXD_E_STATE, "scratch not resolved for this device");
+    const StepScratch sc{base, base + c->nblk_total};
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->step == 0) dispatch_step_end(c, 0, 0, sc.blk_energy);   // row 1: the initial state
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int it = it_begin; it <= it_end; ++it) enqueue_step(c, sc, false);
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipGetLastError());
@@ -725,7 +803,7 @@ int nexd_time_stage_kernel(nexd_ctx *c, int list, int nrep, double *avg_ms, int 
         dispatch_stage(c->cfg.order, a, att, list == 1, c->stream);
     };
     stage_buffers(c, 1, a);
-    if (list == 1 && a.pml_compat) dispatch_last_flux(c, a.qin, a.pin);
+    if (list == 1 && a.pml_compat) dispatch_last_flux(c, a.qin, a.pin, c->last_flux);
     one(1);
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     for (int r = 0; r < nrep; ++r)
