@@ -256,6 +256,21 @@ int nexd_batch_get_energy(nexd_batch *b, int shot, double *energy);
 /* one shot's PML switch for its STA/LTA trigger (src/timeloopMod.f90:532-539),
  * as nexd_set_pml_active for one context; the other shots keep theirs */
 int nexd_batch_set_pml_active(nexd_batch *b, int shot, int active);
+/* Block-shape autotuner (no reference counterpart).  FP32 batches with a
+ * TVD scheme have two shapes of the stage launches: 0 sizes a block for
+ * four blocks per compute unit, 1 for two (twice the LDS, so more elements
+ * per block, and twice the registers per thread).  nexd_batch_autotune times
+ * each on the batch's own element lists (nrep rounds of every RK stage per
+ * list) and keeps the fastest; *tile receives it and ms[0..1] (may be NULL)
+ * the mean stage time of each shape.  Results are cached per kind of batch
+ * for the process.  The timing overwrites the state: the batch is reset
+ * afterwards.  Other batches have one shape (tile 0). */
+int nexd_batch_autotune(nexd_batch *b, int nrep, int *tile, double *ms);
+int nexd_batch_set_tile(nexd_batch *b, int tile);
+/* A step per element (nelem values, the host's double), for instances of
+ * different time step packed into one mesh: each element advances by its
+ * own step (no reference counterpart; the reference has one step per run). */
+int nexd_batch_upload_element_dt(nexd_batch *b, const double *dt_elem);
 int nexd_batch_last_run_ms(nexd_batch *b, double *ms);
 /* measurement: mean duration of one stage launch of element list 0/1,
  * averaged over the RK stages (as nexd_time_stage_kernel); overwrites the

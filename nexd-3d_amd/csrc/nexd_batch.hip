@@ -8,7 +8,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <map>
 #include <memory>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -68,6 +70,8 @@ struct Batch : BatchBase {
     std::vector<int> pml_on_host;
     double *blk_energy = nullptr, *energy = nullptr;   // [shot][nblk_total], [shot][nt] (FP64 sums)
     int nblk_total = 0;
+    int tile = 0;                             // block shape of the stage launches (autotuner)
+    R *dt_elem = nullptr;                     // per-element step, or null (cfg.dt everywhere)
     int step = 0;
     bool have_ops = false, have_mesh = false, initialized = false;
     std::vector<char> have_mat, have_att, have_pml, have_src;
@@ -183,7 +187,7 @@ struct Batch : BatchBase {
 
     // element blocks of one list launch (per shot)
     int nblocks(int n, bool pml) const {
-        return nexd::stage_blocks<R>(cfg.order, n, cfg.attenuation != 0, pml);
+        return nexd::stage_blocks<R>(cfg.order, n, cfg.attenuation != 0, pml, tile);
     }
 
     int rebuild_lists() {
@@ -198,6 +202,12 @@ struct Batch : BatchBase {
         nblk_total = nblocks(n_plain, false) + nblocks(n_pml, true);
         drop_graph();
         return 0;
+    }
+
+    using TileKey = std::tuple<int, int, int, int, int, int, int, int>;
+    static std::map<TileKey, std::pair<int, std::vector<double>>> &tile_cache() {
+        static std::map<TileKey, std::pair<int, std::vector<double>>> c;
+        return c;
     }
 
     // the reference computes the energy after every step of a run with PML
@@ -223,6 +233,7 @@ struct Batch : BatchBase {
         a.nslot = nslot;
         a.scheme = cfg.timeint;
         a.dt = (R)cfg.dt;
+        a.dt_elem = dt_elem;
         a.w13 = f32w ? (R)(1.0f / 3.0f) : (R)(1.0 / 3.0);
         a.w23 = f32w ? (R)(2.0f / 3.0f) : (R)(2.0 / 3.0);
         a.pml_compat = (cfg.compat & NEXD_COMPAT_PML_LAST_ELEMENT) ? 1 : 0;
@@ -247,8 +258,68 @@ struct Batch : BatchBase {
     void dispatch(const nexd::StageArgs<R> &a, bool pml) { dispatch(a, pml, stream); }
     void dispatch(const nexd::StageArgs<R> &a, bool pml, hipStream_t s) {
         const bool att = cfg.attenuation != 0;
-        if (a.scheme == NEXD_LSRK4) nexd::launch_stage<R, true>(cfg.order, att, pml, a, s);
-        else nexd::launch_stage<R, false>(cfg.order, att, pml, a, s);
+        if (a.scheme == NEXD_LSRK4) {
+            nexd::launch_stage<R, true, 0>(cfg.order, att, pml, a, s);
+        } else if (tile == 1) {
+            if constexpr (sizeof(R) == 4) nexd::launch_stage<float, false, 1>(cfg.order, att, pml, a, s);
+        } else {
+            nexd::launch_stage<R, false, 0>(cfg.order, att, pml, a, s);
+        }
+    }
+
+    int nvariants() const { return nexd::stage_variants<R>(cfg.timeint == NEXD_LSRK4); }
+
+    int set_tile(int t) {
+        if (t < 0 || t >= nvariants()) return fail(NEXD_E_ARG, "no such block shape for this precision and scheme");
+        tile = t;
+        nblk_total = nblocks(n_plain, false) + nblocks(n_pml, true);
+        drop_graph();
+        return 0;
+    }
+
+    // Times every block shape on this batch's own element lists (one round
+    // of all RK stages per list, nrep rounds after a warm-up) and keeps the
+    // fastest.  Choices are remembered per (order, attenuation, PML,
+    // precision, shots, elements, PML elements) for the process, so
+    // batches of the same kind reuse them without timing.  The timing
+    // overwrites the state: the batch is reset afterwards.
+    int autotune(int nrep, int *choice, double *ms_out) {
+        const int nv = nvariants();
+        const auto key = std::make_tuple(cfg.order, cfg.attenuation, cfg.pml, cfg.precision, cfg.nshot, cfg.nelem,
+                                         n_pml, cfg.timeint * 2 + (cfg.compensated ? 1 : 0));
+        std::vector<double> ms(nv, 0.0);
+        int best = 0;
+        auto &cache = tile_cache();
+        auto hit = cache.find(key);
+        if (hit != cache.end()) {
+            best = hit->second.first;
+            ms = hit->second.second;
+        } else if (nrk > 1) {
+            if (!initialized) {
+                int rc = reset();
+                if (rc) return rc;
+            }
+            for (int v = 0; v < nv; ++v) {
+                int rc = set_tile(v);
+                if (rc) return rc;
+                for (int list = 0; list < 2; ++list) {
+                    if ((list == 0 && n_plain == 0) || (list == 1 && (!cfg.pml || n_pml == 0))) continue;
+                    double t = 0;
+                    int n = 0;
+                    if ((rc = time_stage(list, nrep, &t, &n))) return rc;
+                    ms[v] += t;
+                }
+                if (ms[v] < ms[best]) best = v;
+            }
+            cache[key] = std::make_pair(best, ms);
+        }
+        int rc = set_tile(best);
+        if (rc) return rc;
+        if ((rc = reset())) return rc;
+        if (choice) *choice = best;
+        if (ms_out)
+            for (int v = 0; v < nv; ++v) ms_out[v] = ms[v];
+        return 0;
     }
 
     template <int ORDER>
@@ -691,6 +762,19 @@ int get_seis(Batch<R> *b, int shot, double *const outs[9]) {
 }
 
 template <typename R>
+int upload_element_dt(Batch<R> *b, const double *dt_elem) {
+    std::vector<R> v(b->cfg.nelem);
+    for (int e = 0; e < b->cfg.nelem; ++e) {
+        if (!(dt_elem[e] > 0)) return b->fail(NEXD_E_ARG, "element steps must be positive");
+        v[e] = (R)dt_elem[e];
+    }
+    int rc = b->upload(&b->dt_elem, v);
+    if (rc) return rc;
+    b->drop_graph();
+    return 0;
+}
+
+template <typename R>
 int get_state(Batch<R> *b, int shot, double *q) {
     if (shot < 0 || shot >= b->cfg.nshot || !q) return b->fail(NEXD_E_ARG, "shot out of range or null output");
     const size_t n = 9 * b->ng;
@@ -941,6 +1025,22 @@ int nexd_batch_get_seismograms(nexd_batch *b, int shot, double *ux, double *uy, 
     if (!b) return NEXD_E_ARG;
     double *const outs[9] = {ux, uy, uz, vx, vy, vz, ax, ay, az};
     return base(b)->cfg.precision == NEXD_FP32 ? get_seis(as<float>(b), shot, outs) : get_seis(as<double>(b), shot, outs);
+}
+
+int nexd_batch_autotune(nexd_batch *b, int nrep, int *tile, double *ms) {
+    if (!b || nrep <= 0) return NEXD_E_ARG;
+    return DISPATCH(b, autotune(nrep, tile, ms));
+}
+
+int nexd_batch_set_tile(nexd_batch *b, int tile) {
+    if (!b) return NEXD_E_ARG;
+    return DISPATCH(b, set_tile(tile));
+}
+
+int nexd_batch_upload_element_dt(nexd_batch *b, const double *dt_elem) {
+    if (!b || !dt_elem) return NEXD_E_ARG;
+    return base(b)->cfg.precision == NEXD_FP32 ? upload_element_dt(as<float>(b), dt_elem)
+                                               : upload_element_dt(as<double>(b), dt_elem);
 }
 
 int nexd_batch_get_state(nexd_batch *b, int shot, double *q) {

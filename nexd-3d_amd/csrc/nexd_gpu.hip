@@ -148,8 +148,8 @@ void drop_graph(nexd_ctx *c) {
 int nblocks(int order, int n, bool att, bool pml) { return nexd::stage_blocks<double>(order, n, att, pml); }
 
 void dispatch_stage(int order, const StageArgs<double> &a, bool att, bool pml, hipStream_t s) {
-    if (a.scheme == NEXD_LSRK4) nexd::launch_stage<double, true>(order, att, pml, a, s);
-    else nexd::launch_stage<double, false>(order, att, pml, a, s);
+    if (a.scheme == NEXD_LSRK4) nexd::launch_stage<double, true, 0>(order, att, pml, a, s);
+    else nexd::launch_stage<double, false, 0>(order, att, pml, a, s);
 }
 
 // Where a step's kernels keep their scratch: the energy block partials
@@ -214,6 +214,7 @@ StageArgs<double> base_args(nexd_ctx *c) {
     a.nslot = c->nslot;
     a.scheme = c->cfg.timeint;
     a.dt = c->cfg.dt;
+    a.dt_elem = nullptr;
     a.w13 = f32w ? (double)(1.0f / 3.0f) : 1.0 / 3.0;
     a.w23 = f32w ? (double)(2.0f / 3.0f) : 2.0 / 3.0;
     a.pml_compat = (c->cfg.compat & NEXD_COMPAT_PML_LAST_ELEMENT) ? 1 : 0;

@@ -56,6 +56,8 @@ struct StageArgs {
     R lsa, lsb;                 // low-storage RK4: A and B of this stage
     int last_stage;             // 1 on the stage that completes a step
     R dt;
+    const R *dt_elem;           // [nelem] the step of each element (instances of different
+                                // step packed in one mesh) or null: dt for all
     R w13, w23;                 // RK3 final-stage weights
     const R *qin;               // [9][nglob] stage input
     const R *qbase;             // [9][nglob] step-start state (RK base); low-storage RK4: the residual
@@ -338,7 +340,10 @@ __device__ constexpr int kPmlCols = 24;
 // precision rounding (its geometry is single precision), which this form
 // averages over the element's nodes: a relative change near 1e-7, below the
 // FP32 path's own error but above the FP64 path's parity bound.
-template <int ORDER, bool ATT, bool PML, typename R>
+// VAR selects one of the block shapes the autotuner times (FP32 only):
+// 0 sizes a block for four blocks per CU, 1 for two, which doubles its LDS
+// share (more elements per block) and its register budget.
+template <int ORDER, bool ATT, bool PML, typename R, int VAR = 0>
 struct Shape {
     static constexpr int NP = Dims<ORDER>::NP, NPF = Dims<ORDER>::NPF, NFP = Dims<ORDER>::NFP;
     static constexpr bool KC = PML && sizeof(R) == 4;          // contravariant volume term
@@ -361,7 +366,7 @@ struct Shape {
     static constexpr int NTHR = 256;
     // blocks per compute unit the shape is sized for (160 KiB of LDS per CU):
     // two FP64 blocks or four FP32 blocks, i.e. 2 or 4 waves per SIMD
-    static constexpr int BPC = sizeof(R) == 8 ? 2 : 4;
+    static constexpr int BPC = sizeof(R) == 8 ? 2 : (VAR ? 2 : 4);
     static constexpr int LDS_BYTES = 160 * 1024 / BPC;
     // waves per SIMD at that occupancy (4 SIMDs per CU): the second
     // __launch_bounds__ argument, which sets the register budget
@@ -450,11 +455,12 @@ struct Mfma<double> {
 };
 
 // LS: the low-storage RK4 instantiation (residuals read and written in
-// place of the RK base); the TVD schemes use LS = false.
-template <int ORDER, bool ATT, bool PML, typename R, bool LS>
-__global__ void __launch_bounds__((Shape<ORDER, ATT, PML, R>::NTHR), (Shape<ORDER, ATT, PML, R>::WPS))
+// place of the RK base); the TVD schemes use LS = false.  VAR: the block
+// shape (Shape).
+template <int ORDER, bool ATT, bool PML, typename R, bool LS, int VAR = 0>
+__global__ void __launch_bounds__((Shape<ORDER, ATT, PML, R, VAR>::NTHR), (Shape<ORDER, ATT, PML, R, VAR>::WPS))
 stage_kernel(StageArgs<R> a) {
-    using S = Shape<ORDER, ATT, PML, R>;
+    using S = Shape<ORDER, ATT, PML, R, VAR>;
     constexpr int NP = S::NP, NPF = S::NPF, NFP = S::NFP, EB = S::EB, NT = S::NT;
     constexpr int NCOL = S::NCOL, NV = S::NV, NFC = S::NFC, NTHR = S::NTHR;
     constexpr int SFN = S::SFN, SLN = S::sln(S::EB);
@@ -525,6 +531,9 @@ stage_kernel(StageArgs<R> a) {
     }
     const size_t g = (size_t)e * NP + j;
     if (node_thr && j == 0) sElem[eb] = e;
+    // the element's step (one value per element; the same for a whole mesh
+    // unless instances of different steps share the launch)
+    const R dt = a.dt_elem ? a.dt_elem[e] : a.dt;
     // first source of the element and its receiver slot, loaded with the
     // first batch instead of at the assembly (read only where valid)
     int src_first = -1, rec_slot_e = -1;
@@ -621,7 +630,7 @@ stage_kernel(StageArgs<R> a) {
                     const R lastv = sLast[c * NP + j];
                     const R u = a.pml_compat ? lastv : mod;
                     const R dadt = -al * c0 - dk * (c0 + u);
-                    pv[i] = rk_update<R, LS>(rs, c0, bse[c], dadt, a.dt);
+                    pv[i] = rk_update<R, LS>(rs, c0, bse[c], dadt, dt);
                 }
                 // one guarded block of stores per direction (a guard per
                 // store leaves waits for every earlier store at the joins)
@@ -1087,7 +1096,7 @@ stage_kernel(StageArgs<R> a) {
                 for (int c = 0; c < 6; ++c) {
                     const int k = i * 6 + c;
                     const R arq = vol[c] + w * lf[9 + c] - w * th[k];
-                    th[k] = rk_update<R, LS, sizeof(R) == 4>(rs, th[k], thb[k], arq, a.dt);
+                    th[k] = rk_update<R, LS, sizeof(R) == 4>(rs, th[k], thb[k], arq, dt);
                 }
             }
             // one guarded block of stores
@@ -1151,14 +1160,14 @@ stage_kernel(StageArgs<R> a) {
 #pragma unroll
             for (int f = 0; f < 9; ++f) {
                 R lo;
-                rk_combine_comp<R>(a.scheme, a.irk, qs[f], alo[f], qb[f], blo[f], rhs[f], a.dt, a.w13, a.w23, qnew[f],
+                rk_combine_comp<R>(a.scheme, a.irk, qs[f], alo[f], qb[f], blo[f], rhs[f], dt, a.w13, a.w23, qnew[f],
                                    lo);
                 if (valid) { qout[f * ng + g] = qnew[f]; a.qlo_out[so + f * ng + g] = lo; }
             }
         } else {
             const RkStage<R> rs = rk_stage<R>(a.scheme, a.irk, a.w13, a.w23, a.lsa, a.lsb);
 #pragma unroll
-            for (int f = 0; f < 9; ++f) qnew[f] = rk_update<R, LS, sizeof(R) == 4>(rs, qs[f], qb[f], rhs[f], a.dt);
+            for (int f = 0; f < 9; ++f) qnew[f] = rk_update<R, LS, sizeof(R) == 4>(rs, qs[f], qb[f], rhs[f], dt);
             if (valid) {
 #pragma unroll
                 for (int f = 0; f < 9; ++f) qout[f * ng + g] = qnew[f];
@@ -1178,7 +1187,7 @@ stage_kernel(StageArgs<R> a) {
         if (valid) {
             if (a.u_full) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) a.u_full[c * ng + g] = a.u_full[c * ng + g] + a.dt * qnew[6 + c];
+                for (int c = 0; c < 3; ++c) a.u_full[c * ng + g] = a.u_full[c * ng + g] + dt * qnew[6 + c];
             }
             const int slot = rec_slot_e;
             if (slot >= 0) {
@@ -1190,10 +1199,10 @@ stage_kernel(StageArgs<R> a) {
                         // displacement by forward Euler, compensated (Kahan)
                         R *ul = a.u_slot_lo + (so * 3 + c) * NP + j;
                         R s, e;
-                        two_sum(*u, a.dt * qnew[6 + c], s, e);
+                        two_sum(*u, dt * qnew[6 + c], s, e);
                         fast_two_sum(s, e + *ul, *u, *ul);
                     } else {
-                        *u = *u + a.dt * qnew[6 + c];
+                        *u = *u + dt * qnew[6 + c];
                     }
                     a.a_slot[(so * 3 + c) * NP + j] = rhs[6 + c];
                 }

@@ -202,6 +202,7 @@ BATCH_EXPORTS = ["nexd_batch_create", "nexd_batch_upload_operators", "nexd_batch
                  "nexd_batch_upload_material", "nexd_batch_upload_anelastic", "nexd_batch_upload_pml",
                  "nexd_batch_upload_sources", "nexd_batch_upload_receivers", "nexd_batch_reset", "nexd_batch_run",
                  "nexd_batch_get_seismograms", "nexd_batch_get_state", "nexd_batch_get_energy", "nexd_batch_set_pml_active",
+                 "nexd_batch_autotune", "nexd_batch_set_tile", "nexd_batch_upload_element_dt",
                  "nexd_batch_last_run_ms", "nexd_batch_time_stage_kernel",
                  "nexd_batch_destroy", "nexd_batch_last_error"]
 GROUP_EXPORTS = ["nexd_group_create", "nexd_group_run", "nexd_group_last_run_ms", "nexd_group_destroy",
@@ -292,6 +293,17 @@ class BatchContext:
         outs = [np.zeros(self.nrec * self.nt) for _ in range(9)]
         self._chk(self.lib.nexd_batch_get_seismograms(self.h, shot, *[_d(o) for o in outs]), "get_seismograms")
         return np.stack([o.reshape(self.nt, self.nrec).T for o in outs], axis=1)
+
+    def autotune(self, nrep: int = 3):
+        """Times the block shapes of the stage launches on this batch and keeps
+        the fastest (resets the state).  Returns (shape, [ms per shape])."""
+        t = ctypes.c_int()
+        ms = (ctypes.c_double * 2)()
+        self._chk(self.lib.nexd_batch_autotune(self.h, nrep, ctypes.byref(t), ms), "autotune")
+        return t.value, [ms[0], ms[1]]
+
+    def set_tile(self, tile: int):
+        self._chk(self.lib.nexd_batch_set_tile(self.h, tile), "set_tile")
 
     def state(self, shot: int) -> np.ndarray:
         """q of one shot as (9, nglob)."""
