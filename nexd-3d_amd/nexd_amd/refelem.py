@@ -83,8 +83,10 @@ _LACZ_P = [f32(v) for v in (0.99999999999980993, 676.5203681218851, -1259.139216
 _PI32 = f32(3.14159265358979324)
 
 
+@lru_cache(maxsize=None)
 def lacz_gamma(a: float) -> float:
-    """Gamma(a) evaluated entirely in float32 as the reference does."""
+    """Gamma(a) evaluated entirely in float32 as the reference does (a pure
+    function of a, memoised)."""
     x = f32(a)
     if x < f32(0.5):
         s = f32(math.sin(float(_PI32 * x)))  # never reached for a >= 1

@@ -8,6 +8,7 @@ src/timeloopMod.f90:284-356 (operands).
 """
 from __future__ import annotations
 
+import functools
 import math
 from dataclasses import dataclass, field
 
@@ -36,6 +37,7 @@ class Located:
     rst: np.ndarray       # (n, 3)
 
 
+@functools.lru_cache(maxsize=None)
 def _search_nodes(order: int):
     """Initial-guess nodes of the search: nodes3D(balpha(NGLL))
     (src/sourceReceiverMod.f90:233-238)."""

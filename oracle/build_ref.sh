@@ -6,6 +6,8 @@
 # into oracle/_ref/:
 #   oracle/_ref/o<N>_<f64|f32>/solver      the reference's program_solver
 #   oracle/_ref/o<N>_<f64|f32>/ref_probe   oracle/ref_probe.f90 (operator dump)
+#   oracle/_ref/o<N>_<f64|f32>/ref_geom_probe  oracle/ref_geom_probe.f90
+#                                          (geometry, location, attenuation)
 #   oracle/_ref/o<N>_<f64|f32>/libnexdref.so  the reference modules
 #
 # Compile-time configuration is the reference's own mechanism: the two
@@ -105,4 +107,7 @@ $FC -o "$OUT/solver" "$SCRATCH/program_solver.o" -L"$OUT" -Wl,-rpath,'$ORIGIN' -
     -Wl,--allow-shlib-undefined
 $FC $FFLAGS -c "$HERE/ref_probe.f90" -o "$SCRATCH/ref_probe.o"
 $FC -o "$OUT/ref_probe" "$SCRATCH/ref_probe.o" -L"$OUT" -Wl,-rpath,'$ORIGIN' -lnexdref $LIBS -Wl,--allow-shlib-undefined
+$FC $FFLAGS -c "$HERE/ref_geom_probe.f90" -o "$SCRATCH/ref_geom_probe.o"
+$FC -o "$OUT/ref_geom_probe" "$SCRATCH/ref_geom_probe.o" -L"$OUT" -Wl,-rpath,'$ORIGIN' -lnexdref $LIBS \
+    -Wl,--allow-shlib-undefined
 echo "built $OUT"
