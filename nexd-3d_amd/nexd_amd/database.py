@@ -178,6 +178,20 @@ def write_source_file(path, sources):
                 f.write(f"{k} = {_v(v)}\n")
 
 
+def write_stf_file(path, sources):
+    """data/stf, the one file every stf=3 source reads (src/stfMod.f90:
+    58-73): rows "time value"; nothing when no source has stf=3."""
+    files = [s.stf_file for s in sources if s.stf == 3]
+    if not files:
+        return
+    rows = np.asarray(files[0], np.float64)
+    if any(not np.array_equal(np.asarray(f, np.float64), rows) for f in files[1:]):
+        raise ValueError("all stf=3 sources read the same data/stf")
+    with open(path, "w") as f:
+        for t, v in rows:
+            f.write(f"{float(t)!r} {float(v)!r}\n")
+
+
 def write_stations(path, xyz):
     with open(path, "w") as f:
         f.write(f"{len(xyz)}\n")
@@ -196,6 +210,7 @@ def write_run_directory(dirname, prob, real="<f8"):
     write_mesh_var(os.path.join(dirname, "out", "meshVar000001"), prob, real)
     if prob.has_src:
         write_src_var(os.path.join(dirname, "out", "srcVar000001"), prob, real)
+        write_stf_file(os.path.join(dirname, "data", "stf"), prob.sources)
     if prob.has_rec:
         write_rec_var(os.path.join(dirname, "out", "recVar000001"), prob, real)
 

@@ -116,17 +116,18 @@ def setup(mesh: meshmod.TetMesh, order: int, par: dict, sources, receivers,
 # ---------------------------------------------------------------------------
 def homogeneous_cube(n: int = 8, order: int = 3, nt: int = 1000, timeint: int = 2,
                      extent: float = 10000.0, nrec: int = 8, cfl: float = 0.25,
-                     f0: float = 1.0, source_kind: int = 0) -> Problem:
+                     f0: float = 1.0, source_kind: int = 0, stf: int = 2, stf_file=None) -> Problem:
     """Plain FP64 homogeneous elastic brick: unit cube of ``extent`` metres,
     n^3 hexes split into 6 tets each, vp/vs/rho = 4000/2310/2700, free top,
     absorbing elsewhere, vertical Ricker point force at the centre, ``nrec``
     receivers on a surface line.  n=8, order=3, nt=1000 is the 3,072-tet
-    N=3 case."""
+    N=3 case.  ``stf``/``stf_file`` choose another source time function
+    (stf=3: the rows (time, value) of data/stf)."""
     m = meshmod.brick_mesh(n, n, n, (extent,) * 3,
                            material=lambda x, y, z: (4000.0, 2310.0, 2700.0, 9000.0, 9000.0))
     c = extent / 2
-    src = srcmod.Source(xyz=(c, c, c), kind=source_kind, stf=2, f0=f0, factor=1.0,
-                        angle_force=(0.0, 0.0, 1.0), M=(1.0, 1.0, 1.0, 0.0, 0.0, 0.0))
+    src = srcmod.Source(xyz=(c, c, c), kind=source_kind, stf=stf, f0=f0, factor=1.0,
+                        angle_force=(0.0, 0.0, 1.0), M=(1.0, 1.0, 1.0, 0.0, 0.0, 0.0), stf_file=stf_file)
     dx = extent / nrec
     recs = [(dx / 2 + dx * k, c, extent) for k in range(nrec)]
     return setup(m, order, dict(timeint=timeint, nt=nt, cfl=cfl), [src], recs,

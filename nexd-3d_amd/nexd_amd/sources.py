@@ -24,11 +24,12 @@ F32 = np.float32
 class Source:
     xyz: tuple
     kind: int = 0                 # 0 single force, 1 moment tensor
-    stf: int = 2                  # 1 Gaussian, 2 Ricker
+    stf: int = 2                  # 1 Gaussian, 2 Ricker, 3 from the samples in stf_file
     f0: float = 1.0
     factor: float = 1.0
     angle_force: tuple = (0.0, 0.0, 1.0)
     M: tuple = (1.0, 1.0, 1.0, 0.0, 0.0, 0.0)   # Mxx Myy Mzz Mxy Mxz Myz
+    stf_file: object = None       # stf=3: (n, 2) rows (time, value), the contents of data/stf
 
 
 @dataclass
@@ -127,16 +128,99 @@ def derivate_stf(time, stf):
     return out
 
 
-def source_time_function(src: Source, nt: int, dt: float) -> np.ndarray:
-    """plotstf(:,2,i) as prepared by the solver (time (it-1)*dt)."""
+def _interval_index(t_old32: np.ndarray, t_new32: float) -> int:
+    """1-based i with t_old(i) <= t_new < t_old(i+1), or n when t_new is at
+    or past the last time (up to one step beyond); the comparisons are in
+    single precision as in the reference (src/stfMod.f90:161-201)."""
+    n = t_old32.size
+    if t_new32 >= t_old32[-1]:
+        if t_new32 > F32(2) * t_old32[-1] - t_old32[-2]:
+            raise ValueError(f"t_new value outside of t_old range: {t_new32} > {t_old32[-1]}")
+        return n
+    i = int(np.searchsorted(t_old32, t_new32, side="right"))
+    if i == 0:
+        raise ValueError(f"index not found: {t_new32} < {t_old32[0]}")
+    return i
+
+
+def cubic_resample(t_old: np.ndarray, y_old: np.ndarray, t_new: np.ndarray) -> np.ndarray:
+    """The cubic through the four samples around each new time
+    (src/stfMod.f90:83-158), in double precision.  A new time whose index
+    matches an old sample within 1e-6 takes that sample.  The four samples
+    start one before the interval holding the time, moved inwards at both
+    ends.  The cubic is formed in monomial coefficients a t^3 + b t^2 + c t
+    + d from the samples' differences, the same arithmetic steps as the
+    reference so that its values agree to the last bit: the caller rounds
+    the result to single precision, where a last-bit difference would show."""
+    t_old = np.asarray(t_old, np.float64)
+    y_old = np.asarray(y_old, np.float64)
+    if np.any(np.diff(t_old) <= 0):
+        raise ValueError("stf file times must increase strictly")
+    n_old = t_old.size
+    if n_old < 4:
+        raise ValueError("stf file needs at least four samples")
+    t32 = t_old.astype(np.float32)
+    out = np.empty(len(t_new))
+    for i, tn in enumerate(np.asarray(t_new, np.float64)):
+        if i < n_old and abs(tn - t_old[i]) < 1e-6:
+            out[i] = y_old[i]
+            continue
+        ind = _interval_index(t32, F32(tn))
+        if ind == 1:
+            ind = 2
+        elif n_old - ind in (0, 1):
+            ind = n_old - 2
+        t1, t2, t3, t4 = t_old[ind - 2:ind + 2]
+        y1, y2, y3, y4 = y_old[ind - 2:ind + 2]
+        y12, y23, y34 = y1 - y2, y2 - y3, y3 - y4
+        t12, t23, t34 = t1 - t2, t2 - t3, t3 - t4
+        t13, t24 = t1 - t3, t2 - t4
+        q1 = t1 * t1 + t1 * t2 - t2 * t3 - t3 * t3
+        q2 = t2 * t2 + t2 * t3 - t3 * t4 - t4 * t4
+        a = (y12 * t23 * t34 * t24 - y23 * t12 * t34 * (t24 + t13) + y34 * t23 * t13 * t12) / (t12 * t23 * t34) / \
+            (q1 * t24 - q2 * t13)
+        b = (y23 * t34 - y34 * t23) / (t23 * t34 * t24) - a * q2 / t24
+        c = y12 / t12 - a * (t1 * t1 + t1 * t2 + t2 * t2) - b * (t1 + t2)
+        # integer powers as products, x**3 = (x*x)*x
+        d = y2 - a * (t2 * t2 * t2) - b * (t2 * t2) - c * t2
+        out[i] = a * (tn * tn * tn) + b * (tn * tn) + c * tn + d
+    return out
+
+
+def stf_from_file(samples, time: np.ndarray) -> np.ndarray:
+    """stf=3 (src/stfMod.f90:47-81): the rows (time, value) of data/stf,
+    their times multiplied by 1000, resampled to the solver's times by
+    ``cubic_resample`` and rounded to single precision (the reference
+    converts the result with real(), whatever its own real kind)."""
+    samples = np.asarray(samples, np.float64)
+    if samples.ndim != 2 or samples.shape[1] != 2:
+        raise ValueError("stf file rows are (time, value)")
+    y = cubic_resample(samples[:, 0] * 1000, samples[:, 1], time)
+    return y.astype(np.float32).astype(np.float64)
+
+
+def stf_samples(src: Source, nt: int, dt: float):
+    """(time, plotstf(:,2,i)) before the moment-rate derivative, as the
+    solver writes them to out/stf%06d (src/sourceReceiverMod.f90:786-810)."""
     time = (np.arange(1, nt + 1, dtype=np.float32) - F32(1.0)).astype(np.float64) * dt
     t0 = float(F32(1.2)) / src.f0
     if src.stf == 1:
         val = -stf_gauss(time, src.f0, t0, src.factor)
     elif src.stf == 2:
         val = -stf_ricker(time, src.f0, t0, src.factor)
+    elif src.stf == 3:
+        if src.stf_file is None:
+            raise ValueError("stf=3 needs the samples of data/stf (Source.stf_file)")
+        val = stf_from_file(src.stf_file, time)
     else:
-        raise ValueError("stf must be 1 (Gaussian) or 2 (Ricker)")
+        raise ValueError("stf must be 1 (Gaussian), 2 (Ricker) or 3 (file)")
+    return time, val
+
+
+def source_time_function(src: Source, nt: int, dt: float) -> np.ndarray:
+    """plotstf(:,2,i) as the time loop uses it: the moment rate for moment
+    tensors (src/sourceReceiverMod.f90:786-812)."""
+    time, val = stf_samples(src, nt, dt)
     if src.kind == 1:
         val = derivate_stf(time, val)
     return val

@@ -3,8 +3,9 @@
 Runs the reference NEXD-3D solver, built from its own sources by
 oracle/build_ref.sh into oracle/_ref/, on one-partition databases written by
 this project's host precompute, and stores the seismograms it writes
-(out/seismo.*) and, for runs with PML, the per-step energy it writes
-(out/energy) as compressed .npz fixtures next to this script.  Also dumps
+(out/seismo.*), the source time function it writes (out/stf000001) and,
+for runs with PML, the per-step energy it writes (out/energy) as compressed
+.npz fixtures next to this script.  Also dumps
 the reference-element operators of the reference (oracle/ref_probe.f90) for
 orders 2..5.
 
@@ -16,6 +17,7 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -42,10 +44,16 @@ def make_case(name: str):
     p = builder(**kw)
     build_ref(p.order)
     t = time.time()
-    seis, out, energy = oracle.run_ref_solver(p, want_energy=True)
-    wall = time.time() - t
+    with tempfile.TemporaryDirectory() as tmp:
+        seis, out, energy = oracle.run_ref_solver(p, workdir=tmp, want_energy=True)
+        wall = time.time() - t
+        # the source time function the reference wrote before the moment-rate
+        # derivative (src/sourceReceiverMod.f90:786-810)
+        stf = np.loadtxt(os.path.join(tmp, "out", "stf000001"), ndmin=2) if p.has_src else None
     switched = "Energy grows too much" in out
     extra = {} if energy is None else {"energy": energy}
+    if stf is not None:
+        extra["stf"] = stf
     np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), seismo=seis, dt=p.dt, nelem=p.nelem,
                         nt=p.nt, order=p.order, wall_s=wall, pml_switched=switched, **extra)
     print(f"{name}: nelem={p.nelem} nt={p.nt} dt={p.dt:.6e} wall={wall:.1f}s switched={switched}", flush=True)

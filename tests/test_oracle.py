@@ -20,7 +20,7 @@ from tests.cases import CASES
 
 TOL = 1e-10
 FAST = ["cube3_n2_rk2", "cube3_n4_rk3", "cube2_n5_rk2", "cube3_n3_moment", "cube3_n3_euler",
-        "layered4_n3_pml", "layered4_n4_att_pml_rk3"]
+        "cube3_n3_stf3", "layered4_n3_pml", "layered4_n4_att_pml_rk3"]
 
 
 def rel_l2(a, b):
