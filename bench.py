@@ -16,8 +16,11 @@ build runs one GPU.
 Other workloads, each printing its own JSON line instead:
   --multishot K  K shots on the 3,072-tet N=3 cube in one FP32 batch against
                  the FP64 plain path, in shot-steps/s;
-  --uneven K     K independent instances of different size and order as one
-                 group against the same instances one by one.
+  --uneven K     K independent instances of different size and order packed
+                 into one union per order, and as a group, against the same
+                 instances one by one;
+  --surrogate    the source-to-seismogram surrogate trained on solver shots,
+                 answering held-out queries against the solver alone.
 """
 from __future__ import annotations
 
@@ -209,10 +212,13 @@ def multishot_main(args):
 
 def uneven_main(args):
     """Uneven batch: K independent instances of different size and order
-    (layered elastic, one point force each), FP32, nt = warmup + steps.  The
-    group (one parent graph per step) is timed against the same instances
-    run one after another."""
-    from nexd_amd import batch, capi
+    (layered elastic, one point force each), FP32, nt = warmup + steps.
+    Three ways are timed: the instances one after another, a group of them
+    (one parent graph per step), and the instances packed into one union per
+    order (each element with its own instance's step, block shape picked by
+    the autotuner) advanced as a group of unions.  ``value`` is the packed
+    rate."""
+    from nexd_amd import batch, capi, uneven
     t0 = time.time()
     inst = batch.uneven_instances(args.uneven, nt=args.warmup + args.steps)
     dof = sum(sb.base.nelem * sb.base.ref.np_ * 9 for sb in inst)
@@ -236,17 +242,94 @@ def uneven_main(args):
         group_ms = g.last_run_ms()
     for m in members:
         m.close()
-    value = dof * args.steps / (group_ms * 1e-3)
-    log(f"[bench] uneven: group {group_ms / args.steps:.3f} ms/step, one by one {alone_ms / args.steps:.3f} ms/step")
+    packs = uneven.pack(inst)
+    packed = [uneven.PackedBatch(pk, precision=capi.NEXD_FP32, autotune=True) for pk in packs]
+    tiles = {str(pk.arrs["order"]): {"members": len(pk.members), "tets": pk.arrs["nelem"], "shape": pb.tile,
+                                      "stage_ms": pb.tile_ms} for pk, pb in zip(packs, packed)}
+    log(f"[bench] packed unions (order: members, tets, chosen block shape, stage ms per shape): {tiles}")
+    for m in packed:
+        m.reset()
+    with capi.Group(packed) as g:
+        if args.warmup:
+            g.run(1, args.warmup)
+        g.run(args.warmup + 1, args.warmup + args.steps)
+        packed_ms = g.last_run_ms()
+    for m in packed:
+        m.close()
+    value = dof * args.steps / (packed_ms * 1e-3)
+    log(f"[bench] uneven: packed {packed_ms / args.steps:.3f} ms/step, group {group_ms / args.steps:.3f} ms/step, "
+        f"one by one {alone_ms / args.steps:.3f} ms/step")
     print(json.dumps({
-        "metric": "DOF-updates/s on 1 GPU, uneven batch (one graph per step vs one by one)",
+        "metric": "DOF-updates/s on 1 GPU, uneven batch (instances packed by order vs one by one)",
         "value": value, "unit": "DOF-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": group_ms / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "ms_per_step": packed_ms / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{args.uneven} instances, 6n^3 tets with n uniform in 10..30, N uniform in "
                                f"{{2,3,4,5}}, 2-4 elastic layers, one point force each, RK2",
                    "dof_per_step": dof, "one_by_one_dof_updates_per_s": dof * args.steps / (alone_ms * 1e-3),
-                   "group_over_one_by_one": alone_ms / group_ms}}), flush=True)
+                   "group_dof_updates_per_s": dof * args.steps / (group_ms * 1e-3),
+                   "group_over_one_by_one": alone_ms / group_ms, "packed_over_one_by_one": alone_ms / packed_ms,
+                   "packed_over_group": group_ms / packed_ms, "packed_unions": tiles}}), flush=True)
+
+
+def surrogate_main(args):
+    """Learned surrogate on the 24,000-tet two-layer anelastic NPML box (20 x
+    20 x 10 hexes, N=3, 512 steps, 16 surface receivers): trained on 4,092
+    solver shots (the six tensor components at 682 positions, a 9 x 9 x 8
+    grid of the source box and 34 random), face band calibrated on 256
+    validation shots, judged on 256 held-out shots (random position, unit
+    tensor, log-uniform M0) solved directly.  ``value`` is held-out queries
+    answered per second by the surrogate path (trust band, spot checks every
+    16th query, fallback on a spot-check error above 2 %), against the solver
+    answering every query."""
+    import numpy as np
+    from nexd_amd import surrogate
+    t0 = time.time()
+    space = surrogate.layered_space()
+    log(f"[bench] surrogate space: {space.base.nelem} tets, nt {space.nt}, {space.nrec} receivers, "
+        f"set-up {time.time() - t0:.1f}s")
+    sur = surrogate.Surrogate(space)
+    tr = sur.train_from_solver(epochs=args.surrogate_epochs, log=log)
+    log(f"[bench] trained: {tr}")
+    fv, mv = surrogate.sample(space, 256, 101)
+    sv, _ = surrogate.solve(space, fv, mv)
+    band, table = sur.calibrate_band(fv, mv, sv)
+    log(f"[bench] face band {band} m; (band, served fraction, p95 error): {table}")
+    fh, mh = surrogate.sample(space, 256, 202)
+    t0 = time.perf_counter()
+    sh, solver_dev = surrogate.solve(space, fh, mh)
+    solver_wall = time.perf_counter() - t0
+    ph = sur.predict(fh, mh)
+    err = surrogate.rel_error(ph, sh)
+    near = surrogate.face_distance(space, fh) < band
+    served = err[~near]
+    ans, info = sur.answer(fh, mh)
+    err_ans = surrogate.rel_error(ans, sh)
+    log(f"[bench] held-out: all {np.median(err):.3e} median, served {len(served)}; answer info {info}")
+    sur.close()
+    q = lambda a, p: float(np.quantile(a, p)) if len(a) else None
+    print(json.dumps({
+        "metric": "held-out shot-seismograms/s answered by the surrogate with solver spot checks and fallback",
+        "value": len(mh) / info["total_s"], "unit": "shot-seismograms/s", "n_gpus": 1, "steps": space.nt,
+        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 solver, bf16 weights / f32 accumulation network", "data": "synthetic",
+        "config": {"workload": "24,000-tet two-layer anelastic NPML box (20x20x10 hexes, N=3), 512 steps, "
+                               "16 surface receivers, sources in a 300x300x200 m box in the upper layer, "
+                               "MLP 9-512x4 (GELU)-24576",
+                   "training": tr, "epochs": args.surrogate_epochs, "face_band_m": band,
+                   "band_table": table,
+                   "held_out": {"shots": len(mh), "error_median_all": q(err, 0.5), "error_p90_all": q(err, 0.9),
+                                "frac_le_2pct_all": float(np.mean(err <= 0.02)),
+                                "served": int(len(served)), "error_median_served": q(served, 0.5),
+                                "error_p95_served": q(served, 0.95),
+                                "error_max_served": float(served.max()) if len(served) else None,
+                                "frac_le_2pct_served": float(np.mean(served <= 0.02)) if len(served) else None,
+                                "set_rel_l2_all": float(np.linalg.norm(ph - sh) / np.linalg.norm(sh))},
+                   "answer": info, "answer_error_max": float(err_ans.max()),
+                   "answer_error_median": float(np.median(err_ans)),
+                   "solver_only_shot_seismograms_per_s": len(mh) / solver_wall,
+                   "solver_only_device_s": solver_dev,
+                   "surrogate_over_solver_only": solver_wall / info["total_s"]}}), flush=True)
 
 
 def main():
@@ -271,6 +354,10 @@ def main():
     ap.add_argument("--uneven", type=int, default=0, metavar="K",
                     help="instead: K uneven instances (6n^3 tets, n in 10..30, N in 2..5, FP32) advanced as one "
                          "group vs one by one")
+    ap.add_argument("--surrogate", action="store_true",
+                    help="instead: train the source-to-seismogram surrogate on solver shots and answer held-out "
+                         "queries with it (spot checks, fallback) vs the solver alone")
+    ap.add_argument("--surrogate-epochs", type=int, default=400)
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
                     help="only summarise rocprofv3 counter runs into per-launch HBM bytes (JSON on stdout)")
     args = ap.parse_args()
@@ -282,6 +369,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "the engine runs on one GPU (multi-GPU is a design, not built); --gpus must be 1"}))
         sys.exit(2)
+    if args.surrogate:
+        surrogate_main(args)
+        return
     if args.uneven:
         uneven_main(args)
         return

@@ -42,9 +42,7 @@ struct Batch : BatchBase {
     int np = 0, npf = 0, nfp = 0, nrk = 0, nslot = 0;
     size_t ng = 0;
     hipStream_t stream = nullptr;
-    hipStream_t side = nullptr;               // interior list during graph capture
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<void *> allocs;
     R *q[3] = {nullptr, nullptr, nullptr};
     R *qres = nullptr, *thres = nullptr, *pres = nullptr;   // low-storage RK4 residuals
@@ -85,9 +83,6 @@ struct Batch : BatchBase {
         for (void *p : allocs) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
-        if (ev_fork) (void)hipEventDestroy(ev_fork);
-        if (ev_join) (void)hipEventDestroy(ev_join);
-        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -139,11 +134,8 @@ struct Batch : BatchBase {
         ng = (size_t)c.nelem * np;
         nrk = c.timeint == 1 ? 1 : (c.timeint == 2 ? 2 : (c.timeint == 3 ? 3 : 5));
         if (hipSetDevice(c.device) != hipSuccess) return fail(NEXD_E_HIP, "hipSetDevice failed");
-        if (hipStreamCreate(&stream) != hipSuccess || hipStreamCreate(&side) != hipSuccess)
-            return fail(NEXD_E_HIP, "hipStreamCreate failed");
-        if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess ||
-            hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&ev_join, hipEventDisableTiming) != hipSuccess)
+        if (hipStreamCreate(&stream) != hipSuccess) return fail(NEXD_E_HIP, "hipStreamCreate failed");
+        if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess)
             return fail(NEXD_E_HIP, "hipEventCreate failed");
         const size_t S = (size_t)c.nshot;
         int rc = 0;
@@ -398,20 +390,16 @@ struct Batch : BatchBase {
         const bool pml = cfg.pml && n_pml > 0;
         for (int irk = 1; irk <= nrk; ++irk) {
             set_stage_buffers(a, irk);
-            // the interior list runs on a side stream, beside the PML list:
-            // both read the stage input and write disjoint elements
-            const bool fork = pml && n_plain > 0;
+            // the interior list, then the PML list, on the one stream (a
+            // captured fork onto a second stream keeps the buffers its
+            // kernels use allocated after the graph and the buffers are
+            // freed, ROCm 7.2; each list fills the device anyway)
             // energy partials of the last stage: [shot][plain blocks, PML blocks]
             const bool en = a.last_stage && energy_on();
-            if (fork) {
-                (void)hipEventRecord(ev_fork, stream);
-                (void)hipStreamWaitEvent(side, ev_fork, 0);
-            }
             a.elist = elist_plain;
             a.nlist = n_plain;
             a.blk_energy = en ? blk_energy : nullptr;
-            dispatch(a, false, fork ? side : stream);
-            if (fork) (void)hipEventRecord(ev_join, side);
+            dispatch(a, false, stream);
             if (pml) {
                 if (a.pml_compat) last_flux_dispatch(a.qin, a.pin);
                 a.elist = elist_pml;
@@ -419,7 +407,6 @@ struct Batch : BatchBase {
                 a.blk_energy = en ? blk_energy + nblocks(n_plain, false) : nullptr;
                 dispatch(a, true);
             }
-            if (fork) (void)hipStreamWaitEvent(stream, ev_join, 0);
             if (cfg.timeint == NEXD_EULER) {
                 (void)hipMemcpyAsync(q[0], q[1], sizeof(R) * 9 * ng * cfg.nshot, hipMemcpyDeviceToDevice, stream);
                 if (cfg.compensated)

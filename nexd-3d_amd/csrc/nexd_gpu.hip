@@ -75,8 +75,6 @@ struct nexd_ctx {
          have_src = false, have_rec = false, initialized = false;
     hipGraphExec_t graph = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    hipStream_t side = nullptr;                 // interior list during graph capture
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     float last_ms = 0.f;
 };
 
@@ -268,9 +266,11 @@ void stage_buffers(nexd_ctx *c, int irk, StageArgs<double> &a) {
     a.lsb = ls ? nexd::kLsrkB[irk - 1] : 0.0;
 }
 
-// one full time step on the context stream; `fork_ok` lets the interior list
-// run on the side stream beside the PML list (graph capture)
-void enqueue_step(nexd_ctx *c, const StepScratch &sc, bool fork_ok) {
+// one full time step on the context stream.  The interior and PML lists are
+// launched one after the other on the one stream: each fills the device, and
+// a captured fork onto a second stream keeps the buffers its kernels use
+// allocated after the graph and the buffers are freed (ROCm 7.2).
+void enqueue_step(nexd_ctx *c, const StepScratch &sc) {
     const int scheme = c->cfg.timeint;
     const bool att = c->cfg.attenuation != 0;
     const bool pml = c->have_pml && c->pml_active && c->n_pml > 0;
@@ -283,19 +283,13 @@ void enqueue_step(nexd_ctx *c, const StepScratch &sc, bool fork_ok) {
         // energy partials: the plain list's blocks first, then the PML list's
         // (every element is in the plain list once the PML is switched off)
         const bool energy = a.last_stage && energy_on(c);
-        // elements without PML (every element when PML is off); with PML they
-        // run on a side stream, beside the PML list: both read the stage
-        // input and write disjoint elements and energy partials
-        const bool fork = fork_ok && pml && c->n_plain > 0;
-        if (fork) {
-            (void)hipEventRecord(c->ev_fork, c->stream);
-            (void)hipStreamWaitEvent(c->side, c->ev_fork, 0);
-        }
+        // elements without PML (every element when PML is off), then the PML
+        // list: both read the stage input and write disjoint elements and
+        // energy partials
         a.elist = c->elist_plain;
         a.nlist = c->n_plain;
         a.blk_energy = energy ? sc.blk_energy : nullptr;
-        dispatch_stage(c->cfg.order, a, att, false, fork ? c->side : c->stream);
-        if (fork) (void)hipEventRecord(c->ev_join, c->side);
+        dispatch_stage(c->cfg.order, a, att, false, c->stream);
         if (pml) {
             if (a.pml_compat) dispatch_last_flux(c, qin, pin, sc.last_flux);
             a.elist = c->elist_pml;
@@ -303,7 +297,6 @@ void enqueue_step(nexd_ctx *c, const StepScratch &sc, bool fork_ok) {
             a.blk_energy = energy ? sc.blk_energy + nblocks(c->cfg.order, c->n_plain, att, false) : nullptr;
             dispatch_stage(c->cfg.order, a, att, true, c->stream);
         }
-        if (fork) (void)hipStreamWaitEvent(c->stream, c->ev_join, 0);
         if (scheme == NEXD_EULER)
             (void)hipMemcpyAsync(c->q[0], c->q[1], sizeof(double) * 9 * c->nglob, hipMemcpyDeviceToDevice, c->stream);
     }
@@ -395,11 +388,8 @@ int nexd_create(const nexd_config *cfg, nexd_ctx **out) {
     c->nrk = cfg->timeint == 1 ? 1 : (cfg->timeint == 2 ? 2 : (cfg->timeint == 3 ? 3 : 5));
     auto bail = [&](int rc) { g_create_error = c->err; nexd_destroy(c); return rc; };
     if (hipSetDevice(cfg->device) != hipSuccess) return bail(fail(c, NEXD_E_HIP, "hipSetDevice failed"));
-    if (hipStreamCreate(&c->stream) != hipSuccess || hipStreamCreate(&c->side) != hipSuccess)
-        return bail(fail(c, NEXD_E_HIP, "hipStreamCreate failed"));
-    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+    if (hipStreamCreate(&c->stream) != hipSuccess) return bail(fail(c, NEXD_E_HIP, "hipStreamCreate failed"));
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
         return bail(fail(c, NEXD_E_HIP, "hipEventCreate failed"));
     {
         const int rs = resolve_scratch(c);
@@ -642,7 +632,7 @@ int nexd_run(nexd_ctx *c, int it_begin, int it_end, double *energy) {
     if (!c->graph) {
         hipGraph_t g = nullptr;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_step(c, StepScratch{c->blk_energy, c->last_flux}, true);
+        enqueue_step(c, StepScratch{c->blk_energy, c->last_flux});
         HIPCHK(c, hipStreamEndCapture(c->stream, &g));
         hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
@@ -679,7 +669,7 @@ int nexd_run_noalloc(nexd_ctx *c, int it_begin, int it_end, double *energy) {
     HIPCHK(c, hipSetDevice(c->cfg.device));
     if (c->step == 0) dispatch_step_end(c, 0, 0, sc.blk_energy);   // row 1: the initial state
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    for (int it = it_begin; it <= it_end; ++it) enqueue_step(c, sc, false);
+    for (int it = it_begin; it <= it_end; ++it) enqueue_step(c, sc);
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipGetLastError());
@@ -827,9 +817,6 @@ void nexd_destroy(nexd_ctx *c) {
     for (auto &a : c->allocs) (void)hipFree(a.first);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
-    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
-    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
-    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -19,7 +19,7 @@ import os
 
 import numpy as np
 
-from . import capi, layout, mesh as meshmod, problem, sources as srcmod
+from . import capi, layout, mesh as meshmod, movie, problem, sources as srcmod
 
 
 def _raw_pairs(path: str):
@@ -54,7 +54,22 @@ def read_parfile(path: str) -> dict:
     return par
 
 
-def read_sources(path: str):
+def read_stf_rows(path: str) -> np.ndarray:
+    """data/stf: rows "time value" read as the reference reads them,
+    list-directed (src/stfMod.f90:58-73)."""
+    rows = []
+    with open(path) as f:
+        for ln in f:
+            parts = ln.replace(",", " ").split()
+            if len(parts) >= 2:
+                rows.append((float(parts[0].replace("d", "e").replace("D", "e")),
+                             float(parts[1].replace("d", "e").replace("D", "e"))))
+    return np.array(rows, dtype=np.float64)
+
+
+def read_sources(path: str, stf_path: str | None = None):
+    """data/source; sources with stf=3 take their samples from ``stf_path``
+    (data/stf)."""
     pairs = _raw_pairs(path)
     nsrc = next(int(_value(v)) for k, v in pairs if k == "nsrc")
     blocks, cur = [], None
@@ -70,6 +85,13 @@ def read_sources(path: str):
                                  stf=int(b["stf"]), f0=float(b["f0"]), factor=float(b["factor"]),
                                  angle_force=(b["xangle_force"], b["yangle_force"], b["zangle_force"]),
                                  M=(b["Mxx"], b["Myy"], b["Mzz"], b["Mxy"], b["Mxz"], b["Myz"])))
+    if any(s.stf == 3 for s in out):
+        if stf_path is None or not os.path.exists(stf_path):
+            raise FileNotFoundError("a source has stf = 3 but data/stf is missing")
+        rows = read_stf_rows(stf_path)
+        for s in out:
+            if s.stf == 3:
+                s.stf_file = rows
     return out
 
 
@@ -111,9 +133,7 @@ def write_stf(outdir: str, p: problem.Problem) -> None:
     785-805)."""
     os.makedirs(outdir, exist_ok=True)
     for i, src in enumerate(p.sources):
-        base = srcmod.Source(**{**src.__dict__, "kind": 0})
-        val = srcmod.source_time_function(base, p.nt, p.dt)
-        t = (np.arange(1, p.nt + 1, dtype=np.float32) - np.float32(1.0)).astype(np.float64) * p.dt
+        t, val = srcmod.stf_samples(src, p.nt, p.dt)
         with open(os.path.join(outdir, f"stf{i + 1:06d}"), "w") as f:
             for k in range(p.nt):
                 f.write(f" {t[k]:.16E} {val[k]:.16E}\n")
@@ -127,7 +147,7 @@ def read_two_columns(path: str) -> np.ndarray:
 def setup_directory(dirname: str, order: int) -> problem.Problem:
     m = meshmod.read_cubit(os.path.join(dirname, "cubit"))
     par = read_parfile(os.path.join(dirname, "data", "parfile"))
-    srcs = read_sources(os.path.join(dirname, "data", "source"))
+    srcs = read_sources(os.path.join(dirname, "data", "source"), os.path.join(dirname, "data", "stf"))
     nr, xyz = read_stations(os.path.join(dirname, "data", "stations"))
     p = problem.setup(m, order, par, srcs, xyz, name=os.path.basename(os.path.abspath(dirname)))
     p.extra["recnr"] = nr
@@ -169,31 +189,44 @@ def sta_lta_fires(energy: np.ndarray, it: int, par: dict, timecrit: int) -> bool
     return bool(abs(sta_lta - 1) > par["sta_lta_trigger"])
 
 
-def advance_with_trigger(ctx: capi.Context, p: problem.Problem):
-    """All nt steps of a run with PML and the reference's energy check:
-    the energy of every step and, once the trigger fires after step k, the
-    PML switched off for steps k+1.. (src/timeloopMod.f90:508-540).  Steps up
-    to timecrit run as one call, later ones one call per step while the
-    trigger is armed.  Returns (energy (nt,), k or 0)."""
+def advance(ctx: capi.Context, p: problem.Problem, on_frame=None):
+    """All nt steps of a run.  With PML, the reference's energy check: the
+    energy of every step and, once the trigger fires after step k, the PML
+    switched off for steps k+1.. (src/timeloopMod.f90:508-540); steps up to
+    timecrit run as one call, later ones one call per step while the
+    trigger is armed.  With ``on_frame``, calls end at every multiple of
+    par["frame"] and on_frame(it) runs there (movie frames,
+    src/timeloopMod.f90:576-582).  Returns (energy (nt,) or None, k or 0)."""
     nt = p.nt
-    energy = np.zeros(nt)
-    timecrit = trigger_start(p)
-    it = 1
-    first = min(nt, timecrit)
-    if first >= 1:
-        energy[:first] = ctx.run(1, first, want_energy=True)
-        it = first + 1
+    pml = bool(p.set_pml)
+    energy = np.zeros(nt) if pml else None
+    timecrit = trigger_start(p) if pml else nt
+    frame = int(p.par.get("frame", 0)) if on_frame is not None else 0
+    if on_frame is not None and frame <= 0:
+        raise ValueError("movie frames need par['frame'] > 0")
     switched = 0
+    it = 1
     while it <= nt:
-        if switched:
-            energy[it - 1:] = ctx.run(it, nt, want_energy=True)
-            break
-        energy[it - 1] = ctx.run(it, it, want_energy=True)[0]
-        if sta_lta_fires(energy, it, p.par, timecrit):
-            ctx.set_pml_active(False)
-            switched = it
-        it += 1
+        end = nt
+        if pml and not switched:
+            end = min(end, max(it, timecrit))
+        if frame:
+            end = min(end, ((it + frame - 1) // frame) * frame)
+        e = ctx.run(it, end, want_energy=pml)
+        if pml:
+            energy[it - 1:end] = e
+            if not switched and sta_lta_fires(energy, end, p.par, timecrit):
+                ctx.set_pml_active(False)
+                switched = end
+        if frame and end % frame == 0:
+            on_frame(end)
+        it = end + 1
     return energy, switched
+
+
+def advance_with_trigger(ctx: capi.Context, p: problem.Problem):
+    """``advance`` of a run with PML, without movie frames."""
+    return advance(ctx, p)
 
 
 def run(dirname: str, order: int, device: int = 0, compat: int = capi.COMPAT_ALL) -> np.ndarray:
@@ -204,16 +237,20 @@ def run(dirname: str, order: int, device: int = 0, compat: int = capi.COMPAT_ALL
     arrs = layout.reference_arrays(p)
     outdir = os.path.join(dirname, "out")
     write_stf(outdir, p)
-    energy = None
-    with capi.Context(arrs, compat=compat, device=device, energy=bool(p.set_pml)) as ctx:
+    want_movie = bool(p.par.get("movie"))
+    with capi.Context(arrs, compat=compat, device=device, energy=bool(p.set_pml),
+                      track_displacement=want_movie) as ctx:
         ctx.reset()
+
+        def frame(it):
+            _, u = ctx.state(displacement=True)
+            movie.write_frame(movie.frame_path(outdir, it), movie.magnitude(u[0], u[1], u[2]))
+
+        energy, switched = advance(ctx, p, on_frame=frame if want_movie else None)
         if p.set_pml:
-            energy, switched = advance_with_trigger(ctx, p)
             p.extra["pml_switched_at"] = switched
             if switched:
                 print(f"PML was switched off at time step {switched}", flush=True)
-        else:
-            ctx.run(1, p.nt)
         seis = ctx.seismograms()
     write_seismograms(outdir, seis, p.dt, p.extra.get("recnr"))
     if energy is not None:

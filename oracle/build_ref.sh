@@ -5,6 +5,7 @@
 # they lie (/root/reference/src), for one polynomial order and one real kind,
 # into oracle/_ref/:
 #   oracle/_ref/o<N>_<f64|f32>/solver      the reference's program_solver
+#   oracle/_ref/o<N>_<f64|f32>/movie       the reference's program_movie
 #   oracle/_ref/o<N>_<f64|f32>/ref_probe   oracle/ref_probe.f90 (operator dump)
 #   oracle/_ref/o<N>_<f64|f32>/ref_geom_probe  oracle/ref_geom_probe.f90
 #                                          (geometry, location, attenuation)
@@ -104,6 +105,9 @@ LIBS="-L/opt/conda/lib -Wl,-rpath,/opt/conda/lib -lmpifort -lmpi -lmkl_rt"
 $FC -shared -o "$OUT/libnexdref.so" $OBJS $LIBS $GPULIB
 $FC $FFLAGS -c "$REF/src/program_solver.f90" -o "$SCRATCH/program_solver.o"
 $FC -o "$OUT/solver" "$SCRATCH/program_solver.o" -L"$OUT" -Wl,-rpath,'$ORIGIN' -lnexdref $LIBS $GPULIB \
+    -Wl,--allow-shlib-undefined
+$FC $FFLAGS -c "$REF/src/program_movie.f90" -o "$SCRATCH/program_movie.o"
+$FC -o "$OUT/movie" "$SCRATCH/program_movie.o" -L"$OUT" -Wl,-rpath,'$ORIGIN' -lnexdref $LIBS $GPULIB \
     -Wl,--allow-shlib-undefined
 $FC $FFLAGS -c "$HERE/ref_probe.f90" -o "$SCRATCH/ref_probe.o"
 $FC -o "$OUT/ref_probe" "$SCRATCH/ref_probe.o" -L"$OUT" -Wl,-rpath,'$ORIGIN' -lnexdref $LIBS -Wl,--allow-shlib-undefined

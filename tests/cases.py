@@ -21,6 +21,14 @@ def cube_file_stf(n: int, order: int, nt: int, timeint: int, source_kind: int):
                                     stf_file=rows)
 
 
+def cube_movie(n: int, order: int, nt: int, timeint: int, frame: int):
+    """The homogeneous cube writing movie frames every ``frame`` steps."""
+    p = problem.homogeneous_cube(n=n, order=order, nt=nt, timeint=timeint)
+    p.par["movie"] = True
+    p.par["frame"] = frame
+    return p
+
+
 CASES = {
     # 3,072-tet homogeneous cube, N=3 (the plain FP64 workload), RK2 and RK3
     "cube8_n3_rk2": (problem.homogeneous_cube, dict(n=8, order=3, nt=1000, timeint=2)),
@@ -33,6 +41,8 @@ CASES = {
     "cube3_n3_euler": (problem.homogeneous_cube, dict(n=3, order=3, nt=50, timeint=1, cfl=0.05)),
     # source time function read from data/stf (stf=3), moment tensor
     "cube3_n3_stf3": (cube_file_stf, dict(n=3, order=3, nt=300, timeint=2, source_kind=1)),
+    # movie frames every 50 steps and their VTK files
+    "cube3_n3_movie": (cube_movie, dict(n=3, order=3, nt=150, timeint=2, frame=50)),
     # reduced two-layer anelastic box with NPML (3,000 tets, N=4)
     "layered10_n4_att_pml": (problem.layered_box, dict(nx=10, ny=10, nz=5, order=4, nt=1000, pml_cells=2,
                                                        nrec_side=3, timeint=2, attenuation=True)),

@@ -3,9 +3,11 @@
 Runs the reference NEXD-3D solver, built from its own sources by
 oracle/build_ref.sh into oracle/_ref/, on one-partition databases written by
 this project's host precompute, and stores the seismograms it writes
-(out/seismo.*), the source time function it writes (out/stf000001) and,
-for runs with PML, the per-step energy it writes (out/energy) as compressed
-.npz fixtures next to this script.  Also dumps
+(out/seismo.*), the source time function it writes (out/stf000001), for
+runs with PML the per-step energy it writes (out/energy), and for runs with
+movie frames the frames and the VTK cell data of the reference's movie
+program (out/moviedata*, out/movie_elem_it*.vtk) as compressed .npz
+fixtures next to this script.  Also dumps
 the reference-element operators of the reference (oracle/ref_probe.f90) for
 orders 2..5.
 
@@ -27,7 +29,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "nexd-3d_amd"))
 sys.path.insert(0, ROOT)
 
-from nexd_amd import problem  # noqa: E402
+from nexd_amd import movie, problem  # noqa: E402
 from oracle import oracle  # noqa: E402
 
 # name -> (builder, kwargs); kept in sync with tests/cases.py
@@ -50,10 +52,23 @@ def make_case(name: str):
         # the source time function the reference wrote before the moment-rate
         # derivative (src/sourceReceiverMod.f90:786-810)
         stf = np.loadtxt(os.path.join(tmp, "out", "stf000001"), ndmin=2) if p.has_src else None
+        if p.par.get("movie"):
+            # the frames the solver wrote and the VTK cell data of the
+            # reference's movie program on them
+            steps = list(range(p.par["frame"], p.nt + 1, p.par["frame"]))
+            frames = np.array([movie.read_frame(movie.frame_path(os.path.join(tmp, "out"), it)) for it in steps])
+            subprocess.run([os.path.join(ROOT, "oracle", "_ref", f"o{p.order}_f64", "movie")], cwd=tmp, check=True,
+                           capture_output=True)
+            vtk = [movie.read_vtk_cells(movie.vtk_path(os.path.join(tmp, "out"), c + 1)) for c in range(len(steps))]
+            movie_extra = {"movie_frames": frames, "movie_steps": np.array(steps), "vtk_points": vtk[0][0],
+                           "vtk_cells": vtk[0][1], "vtk_data": np.array([v[3] for v in vtk])}
+        else:
+            movie_extra = {}
     switched = "Energy grows too much" in out
     extra = {} if energy is None else {"energy": energy}
     if stf is not None:
         extra["stf"] = stf
+    extra.update(movie_extra)
     np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), seismo=seis, dt=p.dt, nelem=p.nelem,
                         nt=p.nt, order=p.order, wall_s=wall, pml_switched=switched, **extra)
     print(f"{name}: nelem={p.nelem} nt={p.nt} dt={p.dt:.6e} wall={wall:.1f}s switched={switched}", flush=True)
