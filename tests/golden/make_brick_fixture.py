@@ -4,19 +4,22 @@ The example holds the mesher inputs (cubit/{coord,mesh,mat,matprop,free,
 absorb}), the run inputs (data/{parfile,source,stations}) and the
 seismograms the reference wrote (ref/seismo.{x,y,z}.%07d.sd{u,v,a}, FP32
 build, 12 ranks, RK2, N=4, 1000 steps).  This script reads those data files
-(as text) and stores their numbers in tests/golden/brick.npz; no reference
-source is involved.
+(as text) and stores their numbers in tests/golden/brick.npz, and the nine
+input files unchanged in tests/golden/brick_inputs.tar.xz (for the tests of
+the readers); no reference source is involved.
 
 Needs /root/reference (this container only).
 """
 import os
 import re
+import tarfile
 
 import numpy as np
 
 REF = os.environ.get("NEXD_REFERENCE", "/root/reference")
 EX = os.path.join(REF, "simulations", "brick")
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "brick.npz")
+TEXT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "brick_inputs.tar.xz")
 
 
 def rows(path, skip_count=True):
@@ -66,6 +69,13 @@ def main():
                         par=np.array([float(re.sub(r"[^0-9.eE+-]", "", par[k])) for k in keys]),
                         source=np.array([float(src[k]) for k in srck]))
     print(f"wrote {OUT}: {elem.shape[0]} tets, {nrec} receivers, {nt} samples")
+    # the input files themselves, as they are, for the readers' tests
+    with tarfile.open(TEXT, "w:xz") as tf:
+        for sub, names in (("cubit", ("coord", "mesh", "mat", "matprop", "free", "absorb")),
+                           ("data", ("parfile", "source", "stations"))):
+            for n in names:
+                tf.add(os.path.join(EX, sub, n), arcname=f"{sub}/{n}")
+    print(f"wrote {TEXT}")
 
 
 if __name__ == "__main__":

@@ -63,3 +63,32 @@ def test_energy_file_round_trip(tmp_path):
     rows = solver.read_two_columns(os.path.join(str(tmp_path), "energy"))
     assert np.allclose(rows[:, 0], 0.004 * np.arange(1, 26), rtol=1e-15)
     assert np.array_equal(rows[:, 1], e)
+
+
+def test_reference_brick_input_files(tmp_path):
+    """The reference's own simulations/brick input files (packed unchanged in
+    tests/golden/brick_inputs.tar.xz) through the readers: the mesh, the
+    parameters, the source and the stations equal the numbers the fixture
+    script read from the same files, and the directory sets up the 9,859-tet
+    N=4 problem with the step of the packed problem."""
+    import tarfile
+    here = os.path.dirname(os.path.abspath(__file__))
+    with tarfile.open(os.path.join(here, "golden", "brick_inputs.tar.xz")) as tf:
+        tf.extractall(str(tmp_path), filter="data")
+    d = np.load(os.path.join(here, "golden", "brick.npz"))
+    m = meshmod.read_cubit(str(tmp_path / "cubit"))
+    assert np.array_equal(m.coord, d["coord"])
+    assert np.array_equal(m.elem + 1, d["elem"])
+    assert np.array_equal(np.column_stack([m.vp, m.vs, m.rho, m.qp, m.qs]), d["props"])
+    assert np.array_equal(m.mat, d["mat"][:, 1]) and np.array_equal(m.pml, d["mat"][:, 2])
+    assert np.array_equal(m.matprop, d["matprop"])
+    assert np.array_equal(m.free + 1, d["free"]) and np.array_equal(m.absorb + 1, d["absorb"])
+    par = solver.read_parfile(str(tmp_path / "data" / "parfile"))
+    assert [par[k] for k in ("timeint", "nt", "cfl", "nproc")] == list(d["par"])
+    (s,) = solver.read_sources(str(tmp_path / "data" / "source"))
+    assert list(s.xyz) + [s.kind, s.stf, s.f0, s.factor] + list(s.angle_force) == list(d["source"])
+    nr, xyz = solver.read_stations(str(tmp_path / "data" / "stations"))
+    assert nr == list(range(1, 12)) and np.array_equal(xyz, d["stations"])
+    p = solver.setup_directory(str(tmp_path), 4)
+    ref = problem.brick_example(os.path.join(here, "golden", "brick.npz"))
+    assert p.nelem == 9859 and p.nt == ref.nt and p.dt == ref.dt
