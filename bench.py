@@ -64,7 +64,8 @@ def b_alg_stage_mean(np_, nv, w, nrk=2, nshot=1):
 NV_PML_ATT = 9 + 18 + 27
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_layered_box_n4.json")
+def traffic_file(order: int) -> str:
+    return os.path.join(ROOT, "profiles", f"hbm_traffic_layered_box_n{order}.json")
 
 
 def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv):
@@ -103,8 +104,8 @@ def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv):
 def cpu_baseline(order=4):
     """The reference solver (oracle/_ref, built from the reference's own
     sources) on a bounded sample of the same workload: the 10x10x5-hex
-    (3,000-tet) version of the two-layer anelastic NPML box, N=4, 20 RK2
-    steps, one MPI rank (1 core).  Its own loop timer is used (set-up
+    (3,000-tet) version of the two-layer anelastic NPML box at the bench's
+    order, 20 RK2 steps, one MPI rank (1 core).  Its own loop timer is used (set-up
     excluded)."""
     from nexd_amd import problem
     from oracle import oracle
@@ -121,7 +122,7 @@ def cpu_baseline(order=4):
     dof = p.nelem * p.ref.np_ * 9 * nt
     return {"value": dof / loop_s, "unit": "DOF-updates/s", "cores": 1, "kind": "reference",
             "sample": f"reference solver (FP64 build, 1 rank) on the 3,000-tet 10x10x5 version of the same "
-                      f"N=4 anelastic NPML box, {nt} RK2 steps, loop time {loop_s:.2f} s (set-up excluded)"}
+                      f"N={order} anelastic NPML box, {nt} RK2 steps, loop time {loop_s:.2f} s (set-up excluded)"}
 
 
 def rel_l2_max(a, b):
@@ -341,8 +342,13 @@ def main():
     ap.add_argument("--nx", type=int, default=40)
     ap.add_argument("--ny", type=int, default=40)
     ap.add_argument("--nz", type=int, default=20)
+    ap.add_argument("--order", type=int, default=4, choices=(2, 3, 4, 5),
+                    help="polynomial order of the box (the headline workload is N=4; other orders profile the "
+                         "stage kernel at low and high order)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--skip-plain-fp32", action="store_true",
+                    help="run only the compensated FP32 batch (counter passes of its kernel)")
     ap.add_argument("--accuracy-nt", type=int, default=4000, metavar="NT",
                     help="also run shot 0 for NT steps (default: the workload's 4000) in FP64 and in FP32 (plain "
                          "and compensated) and report the seismogram error of each FP32 form against FP64; 0 skips")
@@ -363,7 +369,7 @@ def main():
     args = ap.parse_args()
     if args.pmc_summary:
         summ = pmc_summary(*args.pmc_summary)
-        summ["workload"] = f"{args.nx}x{args.ny}x{args.nz}/{args.shots}"
+        summ["workload"] = f"{args.nx}x{args.ny}x{args.nz}/N{args.order}/{args.shots}/compensated"
         print(json.dumps(summ, indent=1))
         return
     if args.gpus != 1:
@@ -394,8 +400,8 @@ def main():
     nt = args.warmup + args.steps + 1
     t0 = time.time()
     pml_cells = 10 if args.nx >= 40 else 2
-    sb = batch.layered_shots(args.shots, nx=args.nx, ny=args.ny, nz=args.nz, order=4, nt=nt, pml_cells=pml_cells,
-                             nrec_side=8, timeint=2, attenuation=True, full_source_arrays=True)
+    sb = batch.layered_shots(args.shots, nx=args.nx, ny=args.ny, nz=args.nz, order=args.order, nt=nt,
+                             pml_cells=pml_cells, nrec_side=8, timeint=2, attenuation=True, full_source_arrays=True)
     p = sb.shots[0]
     log(f"[bench] set-up {time.time() - t0:.1f}s: nelem={p.nelem} N={p.order} PML elements={int(p.mesh.pml.sum())} "
         f"shots={sb.nshot}")
@@ -404,7 +410,10 @@ def main():
 
     np_ = p.ref.np_
     bpe64 = b_alg_stage_mean(np_, NV_PML_ATT, 8)                    # one shot per launch
-    bpe32 = b_alg_stage_mean(np_, NV_PML_ATT, 4, nshot=sb.nshot) / sb.nshot   # per element-shot
+    # per element-shot; the compensated sum also reads and writes the low
+    # parts of the 9 fields (and reads the RK base's on stage 2)
+    bpe32 = b_alg_stage_mean(np_, NV_PML_ATT + 9, 4, nshot=sb.nshot) / sb.nshot
+    bpe32_plain = b_alg_stage_mean(np_, NV_PML_ATT, 4, nshot=sb.nshot) / sb.nshot
 
     # ---- FP64 plain path (the drop-in), shot 0 ------------------------------
     fp64 = None
@@ -429,7 +438,7 @@ def main():
 
     # ---- FP32 batched path, all shots: the compensated RK stage sum and the
     # energy of every step (as the reference computes it for PML runs) -------
-    def fp32_run(compensated, energy):
+    def fp32_run(compensated, energy, bpe):
         b = capi.BatchContext(sb, precision=capi.NEXD_FP32, compat=capi.COMPAT_ALL, compensated=compensated,
                               energy=energy)
         b.reset()
@@ -443,33 +452,36 @@ def main():
         b.close()
         return dict(wall=wall, seis=seis, ms_pml=ms_k, n_pml=n_k, value=dof_step * sb.nshot * args.steps / wall,
                     ms_per_step=wall / args.steps * 1e3,
-                    achieved=bpe32 * n_k * sb.nshot / (ms_k * 1e-3) / 1e9)
+                    achieved=bpe * n_k * sb.nshot / (ms_k * 1e-3) / 1e9)
 
-    f32 = fp32_run(True, True)
+    f32 = fp32_run(True, True, bpe32)
     value32, wall32, ms32_pml, n32_pml, achieved32 = (f32["value"], f32["wall"], f32["ms_pml"], f32["n_pml"],
                                                        f32["achieved"])
     err32 = rel_l2_max(f32["seis"], seis64) if fp64 else None
     log(f"[bench] FP32 batched ({sb.nshot} shots, compensated, energy): {f32['ms_per_step']:.2f} ms/step, PML stage "
         f"kernel {ms32_pml:.3f} ms ({achieved32:.0f} GB/s), seismogram error vs FP64 over {nsteps_all} steps: {err32}")
-    f32u = fp32_run(False, False)
-    log(f"[bench] FP32 batched ({sb.nshot} shots, plain sum, no energy): {f32u['ms_per_step']:.2f} ms/step, PML "
-        f"stage kernel {f32u['ms_pml']:.3f} ms ({f32u['achieved']:.0f} GB/s)")
+    f32u = None
+    if not args.skip_plain_fp32:
+        f32u = fp32_run(False, False, bpe32_plain)
+        log(f"[bench] FP32 batched ({sb.nshot} shots, plain sum, no energy): {f32u['ms_per_step']:.2f} ms/step, PML "
+            f"stage kernel {f32u['ms_pml']:.3f} ms ({f32u['achieved']:.0f} GB/s)")
 
     accuracy = None
     if args.accuracy_nt:
         accuracy = accuracy_run(args, pml_cells)
-    cpu = None if args.no_cpu_baseline else cpu_baseline()
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args.order)
     # HBM bytes per launch measured by rocprofv3 counters on this workload
     # (committed summary, made with --pmc-summary); null when absent
     traffic = traffic64 = None
-    workload_key = f"{args.nx}x{args.ny}x{args.nz}/{sb.nshot}/compensated"
-    if os.path.exists(TRAFFIC_FILE):
-        tf = json.load(open(TRAFFIC_FILE))
+    workload_key = f"{args.nx}x{args.ny}x{args.nz}/N{args.order}/{sb.nshot}/compensated"
+    tfile = traffic_file(args.order)
+    if os.path.exists(tfile):
+        tf = json.load(open(tfile))
         if tf.get("workload") == workload_key:
             for name, v in tf["kernels"].items():
-                if "<4, true, true, float>" in name:
+                if f"<{args.order}, true, true, float>" in name:
                     traffic = v["bytes_per_launch"]
-                elif "<4, true, true, double>" in name:
+                elif f"<{args.order}, true, true, double>" in name:
                     traffic64 = v["bytes_per_launch"]
     out = {
         "metric": "DOF-updates/s on 1 GPU: FP64 plain vs FP32 batched at ≤1e-4 seismogram L2",
@@ -485,7 +497,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"two-layer anelastic NPML box {args.nx}x{args.ny}x{args.nz} hexes "
-                               f"({p.nelem} tets), N=4, 3 SLS, RK2; FP32 batched path with {sb.nshot} shots "
+                               f"({p.nelem} tets), N={args.order}, 3 SLS, RK2; FP32 batched path with {sb.nshot} shots "
                                f"(one moment-tensor source each) in one launch per stage, compensated RK stage "
                                f"sum, energy of every step",
                    "dof_per_step": dof_step * sb.nshot, "rk_stages": 2, "shots": sb.nshot,
@@ -493,21 +505,24 @@ def main():
                    "fp64_plain_ms_per_step": fp64["ms_per_step"] if fp64 else None,
                    "fp32_over_fp64_shot_throughput": (value32 / fp64["value"]) if fp64 else None,
                    "fp32_seismogram_rel_l2_vs_fp64": err32,
-                   "fp32_plain_sum_no_energy": {"dof_updates_per_s": f32u["value"], "ms_per_step": f32u["ms_per_step"],
-                                                "pml_kernel_ms": f32u["ms_pml"],
-                                                "roofline_frac": f32u["achieved"] / HBM_PEAK_GBS},
+                   "fp32_plain_sum_no_energy": ({"dof_updates_per_s": f32u["value"],
+                                                 "ms_per_step": f32u["ms_per_step"], "pml_kernel_ms": f32u["ms_pml"],
+                                                 "bytes_per_element_shot": bpe32_plain,
+                                                 "roofline_frac": f32u["achieved"] / HBM_PEAK_GBS}
+                                                if f32u else None),
                    "accuracy_steps": nsteps_all,
                    "accuracy": accuracy},
         "roofline": {"bound": "hbm", "achieved": achieved32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved32 / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "stage_kernel<4,ATT,PML,f32> (PML elements, all shots, compensated; mean over the 2 "
-                               "RK stages, energy partials on the second)",
+                     "kernel": f"stage_kernel<{args.order},ATT,PML,f32> (PML elements, all shots, compensated; mean "
+                               f"over the 2 RK stages, energy partials on the second)",
                      "bytes_per_element_shot": bpe32,
-                     "bytes_model": "w*Np*54*(2+r) per element-shot (r: RK base read on stage 2) + (54*w + 48) per "
+                     "bytes_model": "w*Np*63*(2+r) per element-shot (9 fields with their low parts, 18 memory "
+                                    "variables, 27 PML auxiliaries; r: RK base read on stage 2) + (54*w + 48) per "
                                     "element shared by the shots",
                      "algorithmic_bytes_per_launch": bpe32 * n32_pml * sb.nshot,
                      "avg_launch_ms": ms32_pml, "elements_per_launch": n32_pml, "shots_per_launch": sb.nshot,
-                     "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if traffic else None,
+                     "traffic_source": os.path.relpath(tfile, ROOT) if traffic else None,
                      "fp64_kernel": ({"achieved": fp64["achieved"], "frac": fp64["achieved"] / HBM_PEAK_GBS,
                                       "avg_launch_ms": fp64["ms_pml"], "elements_per_launch": fp64["n_pml"],
                                       "bytes_per_element": bpe64, "traffic": traffic64}
