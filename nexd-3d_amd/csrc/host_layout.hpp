@@ -98,6 +98,22 @@ inline std::vector<double> mfma_lift_fragments(const std::vector<double> &LT, in
     return out;
 }
 
+// The mass matrix in the A-operand order of the 16x16x4 MFMA (the energy's
+// products M y): row tile mt holds nodes 16mt..16mt+15, k-step ks nodes
+// 4ks..4ks+3; lane l holds M(16mt + l%16, 4ks + l/16), zero beyond Np.  M is
+// [NP][NP] row-major.
+inline std::vector<double> mfma_mass_fragments(const std::vector<double> &M, int np) {
+    const int nt = (np + 15) / 16, nk = (np + 3) / 4;
+    std::vector<double> out((size_t)nt * nk * 64, 0.0);
+    for (int mt = 0; mt < nt; ++mt)
+        for (int ks = 0; ks < nk; ++ks)
+            for (int l = 0; l < 64; ++l) {
+                const int i = 16 * mt + l % 16, k = 4 * ks + l / 16;
+                if (i < np && k < np) out[((size_t)mt * nk + ks) * 64 + l] = M[(size_t)i * np + k];
+            }
+    return out;
+}
+
 struct MeshHost {
     std::vector<double> metric;  // [9][nglob]
     std::vector<double> jac;     // [nglob]

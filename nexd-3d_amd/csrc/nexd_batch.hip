@@ -577,7 +577,7 @@ int upload_ops(Batch<R> *b, const double *dr, const double *ds, const double *dt
     if ((rc = b->upload(&b->D, cvt<R>(nexd::mfma_operator_fragments(o.D, b->np, sizeof(R) == 8)))) ||
         (rc = b->upload(&b->DK, cvt<R>(nexd::mfma_operator_fragments_k(o.D, b->np)))) ||
         (rc = b->upload(&b->LT, cvt<R>(nexd::mfma_lift_fragments(o.LT, b->np, 4 * b->npf)))) ||
-        (rc = b->upload(&b->mass, cvt<R>(o.M))) || (rc = b->upload(&b->fmask, o.fm)))
+        (rc = b->upload(&b->mass, cvt<R>(nexd::mfma_mass_fragments(o.M, b->np)))) || (rc = b->upload(&b->fmask, o.fm)))
         return rc;
     b->fmask_host = o.fm;
     b->have_ops = true;

@@ -437,7 +437,7 @@ int nexd_upload_operators(nexd_ctx *c, const double *dr, const double *ds, const
     if (!err.empty()) return fail(c, NEXD_E_ARG, err);
     int rc;
     if ((rc = upload(c, &c->D, nexd::mfma_operator_fragments(o.D, c->np, true))) ||
-        (rc = upload(c, &c->LT, nexd::mfma_lift_fragments(o.LT, c->np, 4 * c->npf))) || (rc = upload(c, &c->mass, o.M)) ||
+        (rc = upload(c, &c->LT, nexd::mfma_lift_fragments(o.LT, c->np, 4 * c->npf))) || (rc = upload(c, &c->mass, nexd::mfma_mass_fragments(o.M, c->np))) ||
         (rc = upload(c, &c->fmask, o.fm)))
         return rc;
     c->fmask_host = o.fm;

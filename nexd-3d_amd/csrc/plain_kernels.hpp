@@ -99,7 +99,7 @@ struct StageArgs {
     R *u_full;                  // [3][nglob] or null
     double *blk_energy;         // [shot][blk_stride] energy partials (this list's part) or null
     int blk_stride;             // per-shot stride of blk_energy: element blocks of a whole step
-    const R *mass;              // [NP][NP]
+    const R *mass;              // mass matrix as MFMA A fragments [node tile][k-step][64] (energy)
     const R *jac;               // [nglob]
     const R *emat;              // [nelem][2] mu, lambda for the energy
     int emat_stride;            // per-shot stride of emat (0: shared)
@@ -398,7 +398,7 @@ struct Shape {
     }
     static constexpr int lds_reals(int eb) {
         const int m = off_metric(eb) + (KC ? 0 : 9 * eb * NP);
-        const int e = 15 * eb * NP + NTHR;
+        const int e = 24 * eb * NP + 4 * eb;    // energy values and per-element coefficients
         return m > e ? m : e;
     }
     // elements per block: at most one node per thread and 16 tiles of
@@ -1210,20 +1210,32 @@ stage_kernel(StageArgs<R> a) {
         }
         if (a.blk_energy) {
             // energy partial of this block (src/timeloopMod.f90:490-516), summed
-            // in FP64.  The reference's normal "strains" are stress combinations
-            // divided by temp1 = 1/(4 mu^2 + 6 mu lam), that is multiplied by
-            // ce = 4 mu^2 + 6 mu lam; with stresses of a few 1e10 the products
-            // leave the FP32 range, so FP32 carries the combinations through the
-            // mass product and applies ce in FP64.  FP64 follows the reference.
+            // in FP64.  Per element e and field pair: sum over nodes of
+            // x . (M y), with y the velocities and the reference's strain
+            // columns and x the velocities and stresses.  The products M y run
+            // on the matrix cores: M (node tiles of 16, k-steps of 4 nodes,
+            // fragments from a.mass) times the block's EB*9 columns y, one
+            // column per lane; each lane then dots its column with x and
+            // scales by its element's coefficient.  The reference's normal
+            // "strains" are stress combinations divided by temp1 = 1/(4 mu^2 +
+            // 6 mu lam), that is multiplied by ce = 4 mu^2 + 6 mu lam; with
+            // stresses of a few 1e10 the products leave the FP32 range, so FP32
+            // carries the combinations through the mass product and applies
+            // ce in FP64.  FP64 follows the reference.
             constexpr bool F32 = sizeof(R) == 4;
+            constexpr int NKE = (NP + 3) / 4, MTE = (NP + 15) / 16;   // k-steps, node tiles of M
+            constexpr int NCE = EB * 9, CTE = (NCE + 15) / 16;          // columns, column tiles
             __syncthreads();
-            R *sE = lds;   // [15][NT]
-            R lam_e = 0, mu_e = 0;
-            if (valid) {
-                const R *em = a.emat + (size_t)shot * a.emat_stride + (size_t)e * 2;
-                mu_e = em[0]; lam_e = em[1];
-            }
+            R *sE = lds;                  // [15][NT] y (rows 6..14 are the columns' values)
+            R *sX = lds + 15 * NT;        // [9][NT]  x = q after the step
+            R *sEl = lds + 24 * NT;       // [EB][4]  rho, mu, lambda, J of each element
+            static_assert(24 * NT + 4 * EB <= S::LDS, "energy values must fit the LDS");
             if (node_thr) {
+                R lam_e = 0, mu_e = 0;
+                if (valid) {
+                    const R *em = a.emat + (size_t)shot * a.emat_stride + (size_t)e * 2;
+                    mu_e = em[0]; lam_e = em[1];
+                }
                 const R temp1 = R(1.) / (4 * mu_e * mu_e + 6 * mu_e * lam_e);
                 const R temp2 = 2 * mu_e;
                 R y[15];
@@ -1237,37 +1249,62 @@ stage_kernel(StageArgs<R> a) {
                 y[11] = F32 ? c2 : c2 / temp1;
                 y[12] = qnew[3] / temp2; y[13] = qnew[4] / temp2; y[14] = qnew[5] / temp2;
 #pragma unroll
-                for (int c = 0; c < 15; ++c) sE[c * NT + t] = valid ? y[c] : R(0);
+                for (int c = 6; c < 15; ++c) sE[c * NT + t] = valid ? y[c] : R(0);
+#pragma unroll
+                for (int c = 0; c < 9; ++c) sX[c * NT + t] = valid ? qnew[c] : R(0);
+                if (j == 0) {
+                    R *el = sEl + eb * 4;
+                    el[0] = valid ? rho : R(0);
+                    el[1] = mu_e;
+                    el[2] = lam_e;
+                    el[3] = valid ? a.jac[(size_t)e * NP] : R(0);
+                }
             }
             __syncthreads();
             double part = 0;
-            if (valid) {
-                R mv[9];
+            {
+                using V4 = typename Mfma<R>::v4;
+                constexpr int NWE = NTHR / 64, CPWE = (CTE + NWE - 1) / NWE;   // column tiles per wave
+                const int lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
 #pragma unroll
-                for (int c = 0; c < 9; ++c) mv[c] = 0;
-                const int ebase = eb * NP;
-                for (int k = 0; k < NP; ++k) {
-                    const R mjk = a.mass[j * NP + k];
+                for (int cw = 0; cw < CPWE; ++cw) {
+                    const int ct = w + cw * NWE;
+                    if (ct >= CTE) break;
+                    // column kind 0..2: velocities (x = v, y = v), 3..5: normal
+                    // (x = sigma_ii, y = strain combination), 6..8: shear
+                    const int col = ct * 16 + (lane & 15);
+                    const bool cok = col < NCE;
+                    const int ce = cok ? col / 9 : 0, kind = cok ? col - (col / 9) * 9 : 0;
+                    const int xr = kind < 3 ? 6 + kind : kind - 3;
+                    const R *yc = sE + (6 + kind) * NT + ce * NP;
+                    const R *xc = sX + xr * NT + ce * NP;
+                    R bfr[NKE];
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) mv[6 + c] += mjk * sE[(6 + c) * NT + ebase + k];
+                    for (int ks = 0; ks < NKE; ++ks) {
+                        const int m = 4 * ks + (lane >> 4);
+                        bfr[ks] = yc[m < NP ? m : 0];   // M is zero beyond Np
+                    }
+                    double dsum = 0;
 #pragma unroll
-                    for (int c = 0; c < 6; ++c) mv[c] += mjk * sE[(9 + c) * NT + ebase + k];
+                    for (int mt = 0; mt < MTE; ++mt) {
+                        R afr[NKE];
+#pragma unroll
+                        for (int ks = 0; ks < NKE; ++ks) afr[ks] = a.mass[(mt * NKE + ks) * 64 + lane];
+                        V4 acc = {0, 0, 0, 0};
+#pragma unroll
+                        for (int ks = 0; ks < NKE; ++ks) acc = Mfma<R>::fma(afr[ks], bfr[ks], acc);
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const int node = mt * 16 + (sizeof(R) == 8 ? 4 * v + (lane >> 4) : 4 * (lane >> 4) + v);
+                            if (node < NP) dsum += (double)(xc[node] * acc[v]);
+                        }
+                    }
+                    const R *el = sEl + ce * 4;
+                    const double mu = el[1], lam = el[2];
+                    const double coef = kind < 3 ? (double)el[0] : (kind < 6 ? (F32 ? 4 * mu * mu + 6 * mu * lam : 1.0)
+                                                                               : 2.0);
+                    part += cok ? 0.5 * (double)el[3] * coef * dsum : 0.0;
                 }
-                const double jac0 = a.jac[(size_t)e * NP];
-                double d;
-                if constexpr (!F32) {
-                    d = rho * (qnew[6] * mv[6]) + rho * (qnew[7] * mv[7]) + rho * (qnew[8] * mv[8]) +
-                        qnew[0] * mv[0] + qnew[1] * mv[1] + qnew[2] * mv[2] +
-                        2 * (qnew[3] * mv[3] + qnew[4] * mv[4] + qnew[5] * mv[5]);
-                } else {
-                    // the three groups in FP32 (each within its range), combined in FP64
-                    const R dv = qnew[6] * mv[6] + qnew[7] * mv[7] + qnew[8] * mv[8];
-                    const R dn = qnew[0] * mv[0] + qnew[1] * mv[1] + qnew[2] * mv[2];
-                    const R ds = qnew[3] * mv[3] + qnew[4] * mv[4] + qnew[5] * mv[5];
-                    const double mu = mu_e, lam = lam_e;
-                    d = (double)rho * dv + (4 * mu * mu + 6 * mu * lam) * dn + 2.0 * ds;
-                }
-                part = 0.5 * jac0 * d;
             }
             // block sum in a fixed order: within each wave, then over the waves
 #pragma unroll
