@@ -1400,9 +1400,20 @@ step_end_kernel(const R *q, const R *u_slot, const R *a_slot, const R *rec_int, 
         // energy_out[shot][nt]: each shot's partials [shot][nblk] in a fixed order
         for (int shot = 0; shot < nshot; ++shot) {
             const double *pb = blk_energy + (size_t)shot * nblk;
-            double acc = 0;
-            for (int b = threadIdx.x; b < nblk; b += blockDim.x) acc += pb[b];
-            s_red[threadIdx.x] = acc;
+            // eight independent partial sums, so eight loads are in flight per
+            // round trip; combined in a fixed order
+            constexpr int U = 8;
+            double acc[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc[u] = 0;
+            int b = threadIdx.x;
+            const int stride = blockDim.x;
+            for (; b + (U - 1) * stride < nblk; b += U * stride) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) acc[u] += pb[b + u * stride];
+            }
+            for (; b < nblk; b += stride) acc[0] += pb[b];
+            s_red[threadIdx.x] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
             __syncthreads();
             for (int s = blockDim.x / 2; s > 0; s >>= 1) {
                 if ((int)threadIdx.x < s) s_red[threadIdx.x] += s_red[threadIdx.x + s];
