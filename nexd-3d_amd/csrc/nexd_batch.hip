@@ -132,6 +132,9 @@ struct Batch : BatchBase {
         npf = (c.order + 1) * (c.order + 2) / 2;
         nfp = 4 * npf;
         ng = (size_t)c.nelem * np;
+        // the kernels address a field plane (ng values) with 32-bit byte offsets
+        if (ng * sizeof(R) > 0xFFFFFFFFull)
+            return fail(NEXD_E_UNSUPPORTED, "mesh too large: a field plane of nelem*Np values must stay below 4 GiB");
         nrk = c.timeint == 1 ? 1 : (c.timeint == 2 ? 2 : (c.timeint == 3 ? 3 : 5));
         if (hipSetDevice(c.device) != hipSuccess) return fail(NEXD_E_HIP, "hipSetDevice failed");
         if (hipStreamCreate(&stream) != hipSuccess) return fail(NEXD_E_HIP, "hipStreamCreate failed");

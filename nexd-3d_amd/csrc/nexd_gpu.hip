@@ -385,6 +385,12 @@ int nexd_create(const nexd_config *cfg, nexd_ctx **out) {
     c->npf = (cfg->order + 1) * (cfg->order + 2) / 2;
     c->nfp = 4 * c->npf;
     c->nglob = (size_t)cfg->nelem * c->np;
+    // the kernels address a field plane (nglob values) with 32-bit byte offsets
+    if (c->nglob * sizeof(double) > 0xFFFFFFFFull) {
+        g_create_error = "mesh too large: a field plane of nelem*Np doubles must stay below 4 GiB";
+        delete c;
+        return NEXD_E_UNSUPPORTED;
+    }
     c->nrk = cfg->timeint == 1 ? 1 : (cfg->timeint == 2 ? 2 : (cfg->timeint == 3 ? 3 : 5));
     auto bail = [&](int rc) { g_create_error = c->err; nexd_destroy(c); return rc; };
     if (hipSetDevice(cfg->device) != hipSuccess) return bail(fail(c, NEXD_E_HIP, "hipSetDevice failed"));

@@ -198,6 +198,35 @@ __device__ __forceinline__ void upwind_flux(const R *fr, R vp, R vs, R rho, R la
 __device__ __forceinline__ float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_r(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
+// ---------------------------------------------------------------------------
+// Field planes through buffer resources.  A field is a plane of ng values;
+// the plane's base and byte size make one buffer descriptor (built in
+// scalar registers at each access) and the node index is a 32-bit byte
+// offset shared by every plane, so a batch of loads or stores needs no
+// 64-bit address arithmetic in vector registers.  The hardware checks the
+// bound: a read past the plane returns 0 and a write is dropped.  Planes
+// are at most 4 GiB (the host checks ng * sizeof(R)).
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const R *plane, unsigned nbytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<R *>(plane), (short)0, (int)nbytes, 0x00020000);
+}
+template <typename R>
+__device__ __forceinline__ R pload(const R *plane, unsigned nbytes, unsigned idx) {
+    const __amdgpu_buffer_rsrc_t r = plane_rsrc(plane, nbytes);
+    if constexpr (sizeof(R) == 4) return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(idx * 4u), 0, 0));
+    else return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(idx * 8u), 0, 0));
+}
+template <typename R>
+__device__ __forceinline__ void pstore(R *plane, unsigned nbytes, unsigned idx, R v) {
+    const __amdgpu_buffer_rsrc_t r = plane_rsrc(plane, nbytes);
+    if constexpr (sizeof(R) == 4) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)(idx * 4u), 0, 0);
+    else {
+        typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)(idx * 8u), 0, 0);
+    }
+}
+
 // Carpenter and Kennedy's five-stage fourth-order 2N-storage coefficients
 // (NASA TM-109112, 1994), as published
 constexpr double kLsrkA[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
@@ -530,6 +559,8 @@ stage_kernel(StageArgs<R> a) {
         }
     }
     const size_t g = (size_t)e * NP + j;
+    const unsigned g32 = (unsigned)e * NP + j;             // node index (planes are < 4 GiB)
+    const unsigned pnb = (unsigned)(ng * sizeof(R));       // plane size in bytes
     if (node_thr && j == 0) sElem[eb] = e;
     // the element's step (one value per element; the same for a whole mesh
     // unless instances of different steps share the launch)
@@ -554,19 +585,19 @@ stage_kernel(StageArgs<R> a) {
         src_first = a.src_head[(size_t)shot * (a.nglob / NP) + e];
         rec_slot_e = a.rec_slot[e];
 #pragma unroll
-        for (int f = 0; f < 9; ++f) qv[f] = qin[f * ng + g];
+        for (int f = 0; f < 9; ++f) qv[f] = pload(qin + f * ng, pnb, g32);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) mv[k] = a.metric[k * ng + g];
+        for (int k = 0; k < 9; ++k) mv[k] = pload(a.metric + k * ng, pnb, g32);
         const R *mc = matc + (size_t)e * 8;
         m2 = mc[2]; m3 = mc[3]; m4 = mc[4];
         if constexpr (PML) {
             const R *pmlc = a.pmlc + (size_t)shot * a.pmlc_stride_ng * ng;
 #pragma unroll
-            for (int k = 0; k < 9; ++k) pc[k] = pmlc[k * ng + g];
+            for (int k = 0; k < 9; ++k) pc[k] = pload(pmlc + k * ng, pnb, g32);
 #pragma unroll
-            for (int c = 0; c < 27; ++c) cur[c] = pin[c * ng + g];
+            for (int c = 0; c < 27; ++c) cur[c] = pload(pin + c * ng, pnb, g32);
 #pragma unroll
-            for (int c = 0; c < 27; ++c) bse[c] = pbase[c * ng + g];
+            for (int c = 0; c < 27; ++c) bse[c] = pload(pbase + c * ng, pnb, g32);
         }
     }
     if constexpr (PML) __syncthreads();   // the staged fluxes are in LDS
@@ -636,13 +667,13 @@ stage_kernel(StageArgs<R> a) {
                 // store leaves waits for every earlier store at the joins)
                 if (valid) {
 #pragma unroll
-                    for (int i = 0; i < 9; ++i) pout[(d * 9 + i) * ng + g] = pv[i];
+                    for (int i = 0; i < 9; ++i) pstore(pout + (d * 9 + i) * ng, pnb, g32, pv[i]);
                 }
                 if constexpr (LS) {
                     if (valid) {
                         R *pres = a.pres + (size_t)shot * 27 * ng;
 #pragma unroll
-                        for (int i = 0; i < 9; ++i) pres[(d * 9 + i) * ng + g] = bse[d * 9 + i];
+                        for (int i = 0; i < 9; ++i) pstore(pres + (d * 9 + i) * ng, pnb, g32, bse[d * 9 + i]);
                     }
                 }
                 // the nine updates of a direction interleave; a scheduling
@@ -683,11 +714,11 @@ stage_kernel(StageArgs<R> a) {
         // the neighbour trace is loaded unconditionally (node 0 stands in on
         // boundary faces) so the nine loads issue together
         const int nb = a.nbr[((size_t)fe * 4 + fi) * NPF + i];
-        const size_t gn = nb >= 0 ? (size_t)nb : 0;
+        const unsigned gn = nb >= 0 ? (unsigned)nb : 0u;
         R dq[9];
 #pragma unroll
         for (int f = 0; f < 9; ++f) {
-            const R qn = qin[f * ng + gn], qo = sQ[f * NT + own];
+            const R qn = pload(qin + f * ng, pnb, gn), qo = sQ[f * NT + own];
             dq[f] = ft >= 0 ? qn - qo : qo;
         }
         const R *mc = matc + (size_t)fe * 8;
@@ -715,7 +746,7 @@ stage_kernel(StageArgs<R> a) {
                 const R *av = a.anel + (size_t)shot * a.anel_stride + (size_t)e * NMB * 3;
 #pragma unroll
                 for (int c = 0; c < 18; ++c) {
-                    const R v = thin[c * ng + g];
+                    const R v = pload(thin + c * ng, pnb, g32);
                     th[c] = valid ? v : R(0);
                 }
 #pragma unroll
@@ -1002,7 +1033,7 @@ stage_kernel(StageArgs<R> a) {
             } else {
 #pragma unroll
                 for (int c = 0; c < 18; ++c) {
-                    const R v = thbase[c * ng + g];
+                    const R v = pload(thbase + c * ng, pnb, g32);
                     thb[c] = valid ? v : R(0);
                 }
             }
@@ -1013,7 +1044,7 @@ stage_kernel(StageArgs<R> a) {
         } else {
 #pragma unroll
             for (int f = 0; f < 9; ++f) {
-                const R v = qbase[f * ng + g];
+                const R v = pload(qbase + f * ng, pnb, g32);
                 qb[f] = valid ? v : R(0);
             }
         }
@@ -1102,13 +1133,13 @@ stage_kernel(StageArgs<R> a) {
             // one guarded block of stores
             if (valid) {
 #pragma unroll
-                for (int k = 0; k < 18; ++k) thout[k * ng + g] = th[k];
+                for (int k = 0; k < 18; ++k) pstore(thout + k * ng, pnb, g32, th[k]);
             }
             if constexpr (LS) {
                 if (valid) {
                     R *thres = a.thres + (size_t)shot * 18 * ng;
 #pragma unroll
-                    for (int k = 0; k < 18; ++k) thres[k * ng + g] = thb[k];
+                    for (int k = 0; k < 18; ++k) pstore(thres + k * ng, pnb, g32, thb[k]);
                 }
             }
         }
@@ -1149,12 +1180,12 @@ stage_kernel(StageArgs<R> a) {
             R alo[9], blo[9];
 #pragma unroll
             for (int f = 0; f < 9; ++f) {
-                const R v = a.qlo_in[so + f * ng + g];
+                const R v = pload(a.qlo_in + so + f * ng, pnb, g32);
                 alo[f] = valid ? v : R(0);
             }
 #pragma unroll
             for (int f = 0; f < 9; ++f) {
-                const R v = a.qlo_base[so + f * ng + g];
+                const R v = pload(a.qlo_base + so + f * ng, pnb, g32);
                 blo[f] = (a.irk == 1 || !valid) ? alo[f] : v;
             }
 #pragma unroll
@@ -1162,7 +1193,7 @@ stage_kernel(StageArgs<R> a) {
                 R lo;
                 rk_combine_comp<R>(a.scheme, a.irk, qs[f], alo[f], qb[f], blo[f], rhs[f], dt, a.w13, a.w23, qnew[f],
                                    lo);
-                if (valid) { qout[f * ng + g] = qnew[f]; a.qlo_out[so + f * ng + g] = lo; }
+                if (valid) { pstore(qout + f * ng, pnb, g32, qnew[f]); pstore(a.qlo_out + so + f * ng, pnb, g32, lo); }
             }
         } else {
             const RkStage<R> rs = rk_stage<R>(a.scheme, a.irk, a.w13, a.w23, a.lsa, a.lsb);
@@ -1170,13 +1201,13 @@ stage_kernel(StageArgs<R> a) {
             for (int f = 0; f < 9; ++f) qnew[f] = rk_update<R, LS, sizeof(R) == 4>(rs, qs[f], qb[f], rhs[f], dt);
             if (valid) {
 #pragma unroll
-                for (int f = 0; f < 9; ++f) qout[f * ng + g] = qnew[f];
+                for (int f = 0; f < 9; ++f) pstore(qout + f * ng, pnb, g32, qnew[f]);
             }
             if constexpr (LS) {
                 if (valid) {
                     R *qres = a.qres + (size_t)shot * 9 * ng;
 #pragma unroll
-                    for (int f = 0; f < 9; ++f) qres[f * ng + g] = qb[f];
+                    for (int f = 0; f < 9; ++f) pstore(qres + f * ng, pnb, g32, qb[f]);
                 }
             }
         }
