@@ -7,6 +7,8 @@
 // (the same kernels as the single-shot path, instantiated for the type).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cstring>
 #include <map>
 #include <memory>
@@ -133,8 +135,10 @@ struct Batch : BatchBase {
         nfp = 4 * npf;
         ng = (size_t)c.nelem * np;
         // the kernels address a field plane (ng values) with 32-bit byte offsets
-        if (ng * sizeof(R) > 0xFFFFFFFFull)
-            return fail(NEXD_E_UNSUPPORTED, "mesh too large: a field plane of nelem*Np values must stay below 4 GiB");
+        const size_t per_elem = std::max<size_t>(std::max<size_t>((size_t)np, (size_t)nfp), 36);
+        if ((size_t)c.nelem * per_elem * sizeof(R) > 0xFFFFFFFFull)
+            return fail(NEXD_E_UNSUPPORTED, "mesh too large: a field plane (nelem*Np) or per-element face array "
+                                            "(nelem*max(4NpF, 36)) must stay below 4 GiB");
         nrk = c.timeint == 1 ? 1 : (c.timeint == 2 ? 2 : (c.timeint == 3 ? 3 : 5));
         if (hipSetDevice(c.device) != hipSuccess) return fail(NEXD_E_HIP, "hipSetDevice failed");
         if (hipStreamCreate(&stream) != hipSuccess) return fail(NEXD_E_HIP, "hipStreamCreate failed");

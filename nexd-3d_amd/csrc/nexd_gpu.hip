@@ -6,6 +6,8 @@
 // of one time step (all RK stages plus the step-end kernel) once per step.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -386,8 +388,10 @@ int nexd_create(const nexd_config *cfg, nexd_ctx **out) {
     c->nfp = 4 * c->npf;
     c->nglob = (size_t)cfg->nelem * c->np;
     // the kernels address a field plane (nglob values) with 32-bit byte offsets
-    if (c->nglob * sizeof(double) > 0xFFFFFFFFull) {
-        g_create_error = "mesh too large: a field plane of nelem*Np doubles must stay below 4 GiB";
+    const size_t per_elem = std::max<size_t>(std::max<size_t>(c->np, c->nfp), 36);   // field plane, fscale, frames
+    if ((size_t)cfg->nelem * per_elem * sizeof(double) > 0xFFFFFFFFull) {
+        g_create_error = "mesh too large: a field plane (nelem*Np) or per-element face array (nelem*max(4NpF, 36)) "
+                         "of doubles must stay below 4 GiB";
         delete c;
         return NEXD_E_UNSUPPORTED;
     }

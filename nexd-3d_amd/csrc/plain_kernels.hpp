@@ -561,6 +561,7 @@ stage_kernel(StageArgs<R> a) {
     const size_t g = (size_t)e * NP + j;
     const unsigned g32 = (unsigned)e * NP + j;             // node index (planes are < 4 GiB)
     const unsigned pnb = (unsigned)(ng * sizeof(R));       // plane size in bytes
+    const unsigned nel = (unsigned)(ng / NP);              // elements of the mesh (per-element arrays)
     if (node_thr && j == 0) sElem[eb] = e;
     // the element's step (one value per element; the same for a whole mesh
     // unless instances of different steps share the launch)
@@ -588,8 +589,10 @@ stage_kernel(StageArgs<R> a) {
         for (int f = 0; f < 9; ++f) qv[f] = pload(qin + f * ng, pnb, g32);
 #pragma unroll
         for (int k = 0; k < 9; ++k) mv[k] = pload(a.metric + k * ng, pnb, g32);
-        const R *mc = matc + (size_t)e * 8;
-        m2 = mc[2]; m3 = mc[3]; m4 = mc[4];
+        const unsigned mb = nel * 8u * (unsigned)sizeof(R);
+        m2 = pload(matc, mb, (unsigned)e * 8 + 2);
+        m3 = pload(matc, mb, (unsigned)e * 8 + 3);
+        m4 = pload(matc, mb, (unsigned)e * 8 + 4);
         if constexpr (PML) {
             const R *pmlc = a.pmlc + (size_t)shot * a.pmlc_stride_ng * ng;
 #pragma unroll
@@ -709,11 +712,11 @@ stage_kernel(StageArgs<R> a) {
         }
         const int fe = sElem[fb];
         const int fi = k / NPF, i = k - fi * NPF;
-        const int own = fb * NP + a.fmask[fi * NPF + i];
-        const int ft = a.ftype[fe * 4 + fi];
+        const int own = fb * NP + pload(a.fmask, 4u * NPF * 4u, (unsigned)(fi * NPF + i));
+        const int ft = pload(a.ftype, nel * 16u, (unsigned)fe * 4 + fi);
         // the neighbour trace is loaded unconditionally (node 0 stands in on
         // boundary faces) so the nine loads issue together
-        const int nb = a.nbr[((size_t)fe * 4 + fi) * NPF + i];
+        const int nb = pload(a.nbr, nel * (unsigned)(NFP * 4), ((unsigned)fe * 4 + fi) * NPF + i);
         const unsigned gn = nb >= 0 ? (unsigned)nb : 0u;
         R dq[9];
 #pragma unroll
@@ -721,11 +724,16 @@ stage_kernel(StageArgs<R> a) {
             const R qn = pload(qin + f * ng, pnb, gn), qo = sQ[f * NT + own];
             dq[f] = ft >= 0 ? qn - qo : qo;
         }
-        const R *mc = matc + (size_t)fe * 8;
+        const unsigned mb = nel * 8u * (unsigned)sizeof(R);
+        R mc[5], fr[9];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) mc[c] = pload(matc, mb, (unsigned)fe * 8 + c);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) fr[c] = pload(a.frame, nel * (unsigned)(36 * sizeof(R)), ((unsigned)fe * 4 + fi) * 9 + c);
         R fl[9], af[6];
-        upwind_flux<R>(a.frame + ((size_t)fe * 4 + fi) * 9, mc[0], mc[1], mc[2], mc[3], mc[4], ft == -1, dq, fl, af, ATT);
+        upwind_flux<R>(fr, mc[0], mc[1], mc[2], mc[3], mc[4], ft == -1, dq, fl, af, ATT);
         const R sgn = ft == -2 ? R(-1) : R(1);
-        const R fs = a.fscale[(size_t)fe * NFP + k];
+        const R fs = pload(a.fscale, nel * (unsigned)(NFP * sizeof(R)), (unsigned)fe * NFP + k);
 #pragma unroll
         for (int f = 0; f < 9; ++f) dst[f * SFN] = fs * (sgn * fl[f]) / 2;
         if constexpr (ATT) {
@@ -743,7 +751,7 @@ stage_kernel(StageArgs<R> a) {
         if constexpr (ATT) {
             if (node_thr) {
                 const R *thin = a.thin + (size_t)shot * 18 * ng;
-                const R *av = a.anel + (size_t)shot * a.anel_stride + (size_t)e * NMB * 3;
+                const R *av = a.anel + (size_t)shot * a.anel_stride;
 #pragma unroll
                 for (int c = 0; c < 18; ++c) {
                     const R v = pload(thin + c * ng, pnb, g32);
@@ -751,7 +759,7 @@ stage_kernel(StageArgs<R> a) {
                 }
 #pragma unroll
                 for (int c = 0; c < 9; ++c) {
-                    const R v = av[c];
+                    const R v = pload(av, nel * (unsigned)(NMB * 3 * sizeof(R)), (unsigned)e * (NMB * 3) + c);
                     avc[c] = valid ? v : R(0);
                 }
             }
