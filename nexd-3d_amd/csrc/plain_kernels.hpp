@@ -562,6 +562,10 @@ stage_kernel(StageArgs<R> a) {
     const unsigned g32 = (unsigned)e * NP + j;             // node index (planes are < 4 GiB)
     const unsigned pnb = (unsigned)(ng * sizeof(R));       // plane size in bytes
     const unsigned nel = (unsigned)(ng / NP);              // elements of the mesh (per-element arrays)
+    // the RK base planes: stage 1 does not use the base (its weight is 0, or
+    // A_1 = 0 for the low-storage residual), so their descriptors get size 0
+    // there and the loads return 0 without touching memory
+    const unsigned pnb_base = a.irk == 1 ? 0u : pnb;
     if (node_thr && j == 0) sElem[eb] = e;
     // the element's step (one value per element; the same for a whole mesh
     // unless instances of different steps share the launch)
@@ -573,8 +577,8 @@ stage_kernel(StageArgs<R> a) {
     // Phase 1 issues every load of the node threads in one batch: q, metric
     // and material and (PML) the coefficients and all 27 auxiliaries and RK
     // bases.  No load is guarded by a branch: padding threads read element
-    // 0, and in stage 1 the RK base buffer is the input buffer (the value is
-    // not used).
+    // 0, and in stage 1 the RK base loads return 0 (their descriptors have
+    // size 0; the value is not used).
     R qs[9];
     R ef0 = 0, ef1 = 0, ef2 = 0, ef3 = 0, rho = 1;
     R qv[9], mv[9], m2 = 1, m3 = 0, m4 = 0;
@@ -600,7 +604,7 @@ stage_kernel(StageArgs<R> a) {
 #pragma unroll
             for (int c = 0; c < 27; ++c) cur[c] = pload(pin + c * ng, pnb, g32);
 #pragma unroll
-            for (int c = 0; c < 27; ++c) bse[c] = pload(pbase + c * ng, pnb, g32);
+            for (int c = 0; c < 27; ++c) bse[c] = pload(pbase + c * ng, pnb_base, g32);
         }
     }
     if constexpr (PML) __syncthreads();   // the staged fluxes are in LDS
@@ -1193,7 +1197,7 @@ stage_kernel(StageArgs<R> a) {
             }
 #pragma unroll
             for (int f = 0; f < 9; ++f) {
-                const R v = pload(a.qlo_base + so + f * ng, pnb, g32);
+                const R v = pload(a.qlo_base + so + f * ng, pnb_base, g32);
                 blo[f] = (a.irk == 1 || !valid) ? alo[f] : v;
             }
 #pragma unroll
