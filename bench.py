@@ -479,9 +479,9 @@ def main():
         tf = json.load(open(tfile))
         if tf.get("workload") == workload_key:
             for name, v in tf["kernels"].items():
-                if f"<{args.order}, true, true, float>" in name:
+                if f"<{args.order}, true, true, float," in name:
                     traffic = v["bytes_per_launch"]
-                elif f"<{args.order}, true, true, double>" in name:
+                elif f"<{args.order}, true, true, double," in name:
                     traffic64 = v["bytes_per_launch"]
     out = {
         "metric": "DOF-updates/s on 1 GPU: FP64 plain vs FP32 batched at ≤1e-4 seismogram L2",
