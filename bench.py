@@ -187,6 +187,8 @@ def multishot_main(args):
         ms64 = ctx.last_run_ms() / args.steps
         ref = ctx.seismograms()
     with capi.BatchContext(sb, precision=capi.NEXD_FP32, compat=capi.COMPAT_ALL) as b:
+        tile, tile_ms = b.autotune(nrep=5)
+        log(f"[bench] multishot: block shape {tile} (stage ms per shape {tile_ms})")
         b.reset()
         if args.warmup:
             b.run(1, args.warmup)
@@ -208,7 +210,8 @@ def multishot_main(args):
                                f"velocity perturbation and random moment tensor per shot, 64 receivers, RK2",
                    "dof_per_step": dof * sb.nshot, "dof_updates_per_s": dof * rate32,
                    "fp64_plain_shot_steps_per_s": rate64, "fp32_over_fp64_shot_throughput": rate32 / rate64,
-                   "fp32_seismogram_rel_l2_vs_fp64": err}}), flush=True)
+                   "fp32_seismogram_rel_l2_vs_fp64": err, "fp32_block_shape": tile,
+                   "fp32_stage_ms_per_shape": tile_ms}}), flush=True)
 
 
 def uneven_main(args):

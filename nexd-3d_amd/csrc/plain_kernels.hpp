@@ -211,19 +211,32 @@ template <typename R>
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const R *plane, unsigned nbytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<R *>(plane), (short)0, (int)nbytes, 0x00020000);
 }
-template <typename R>
+// BUF selects the buffer form; without it the same accesses are plain
+// global loads and stores (each stage kernel instantiation picks the form
+// measured faster for it).
+template <bool BUF = true, typename R>
 __device__ __forceinline__ R pload(const R *plane, unsigned nbytes, unsigned idx) {
-    const __amdgpu_buffer_rsrc_t r = plane_rsrc(plane, nbytes);
-    if constexpr (sizeof(R) == 4) return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(idx * 4u), 0, 0));
-    else return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(idx * 8u), 0, 0));
+    if constexpr (!BUF) {
+        (void)nbytes;
+        return plane[idx];
+    } else {
+        const __amdgpu_buffer_rsrc_t r = plane_rsrc(plane, nbytes);
+        if constexpr (sizeof(R) == 4) return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(idx * 4u), 0, 0));
+        else return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(idx * 8u), 0, 0));
+    }
 }
-template <typename R>
+template <bool BUF = true, typename R>
 __device__ __forceinline__ void pstore(R *plane, unsigned nbytes, unsigned idx, R v) {
-    const __amdgpu_buffer_rsrc_t r = plane_rsrc(plane, nbytes);
-    if constexpr (sizeof(R) == 4) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)(idx * 4u), 0, 0);
-    else {
-        typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)(idx * 8u), 0, 0);
+    if constexpr (!BUF) {
+        (void)nbytes;
+        plane[idx] = v;
+    } else {
+        const __amdgpu_buffer_rsrc_t r = plane_rsrc(plane, nbytes);
+        if constexpr (sizeof(R) == 4) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)(idx * 4u), 0, 0);
+        else {
+            typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)(idx * 8u), 0, 0);
+        }
     }
 }
 
@@ -496,6 +509,11 @@ stage_kernel(StageArgs<R> a) {
     __shared__ __align__(16) R lds[S::LDS];
     __shared__ int sElem[EB];             // the block's element ids, for the face phase
     constexpr bool KC = S::KC;
+    // field planes through buffer descriptors, where measured faster: with
+    // PML or attenuation, in the two-blocks-per-CU shape and at N = 5; the
+    // plain elastic kernels of the four-blocks-per-CU shape up to N = 4 ran
+    // 10-25 % slower with them (their few loads were scheduled in series)
+    constexpr bool BUF = PML || ATT || VAR == 1 || ORDER >= 5;
     R *sQ = lds;                          // [9][NT]
     R *sX = lds + 9 * NT;                 // [24][NT]    PML flux columns (FP64)
     R *sC = lds + 9 * NT;                 // [3][12][NT] C_o,i and o_k, contravariant form (FP32 PML)
@@ -590,21 +608,21 @@ stage_kernel(StageArgs<R> a) {
         src_first = a.src_head[(size_t)shot * (a.nglob / NP) + e];
         rec_slot_e = a.rec_slot[e];
 #pragma unroll
-        for (int f = 0; f < 9; ++f) qv[f] = pload(qin + f * ng, pnb, g32);
+        for (int f = 0; f < 9; ++f) qv[f] = pload<BUF>(qin + f * ng, pnb, g32);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) mv[k] = pload(a.metric + k * ng, pnb, g32);
+        for (int k = 0; k < 9; ++k) mv[k] = pload<BUF>(a.metric + k * ng, pnb, g32);
         const unsigned mb = nel * 8u * (unsigned)sizeof(R);
-        m2 = pload(matc, mb, (unsigned)e * 8 + 2);
-        m3 = pload(matc, mb, (unsigned)e * 8 + 3);
-        m4 = pload(matc, mb, (unsigned)e * 8 + 4);
+        m2 = pload<BUF>(matc, mb, (unsigned)e * 8 + 2);
+        m3 = pload<BUF>(matc, mb, (unsigned)e * 8 + 3);
+        m4 = pload<BUF>(matc, mb, (unsigned)e * 8 + 4);
         if constexpr (PML) {
             const R *pmlc = a.pmlc + (size_t)shot * a.pmlc_stride_ng * ng;
 #pragma unroll
-            for (int k = 0; k < 9; ++k) pc[k] = pload(pmlc + k * ng, pnb, g32);
+            for (int k = 0; k < 9; ++k) pc[k] = pload<BUF>(pmlc + k * ng, pnb, g32);
 #pragma unroll
-            for (int c = 0; c < 27; ++c) cur[c] = pload(pin + c * ng, pnb, g32);
+            for (int c = 0; c < 27; ++c) cur[c] = pload<BUF>(pin + c * ng, pnb, g32);
 #pragma unroll
-            for (int c = 0; c < 27; ++c) bse[c] = pload(pbase + c * ng, pnb_base, g32);
+            for (int c = 0; c < 27; ++c) bse[c] = pload<BUF>(pbase + c * ng, pnb_base, g32);
         }
     }
     if constexpr (PML) __syncthreads();   // the staged fluxes are in LDS
@@ -674,13 +692,13 @@ stage_kernel(StageArgs<R> a) {
                 // store leaves waits for every earlier store at the joins)
                 if (valid) {
 #pragma unroll
-                    for (int i = 0; i < 9; ++i) pstore(pout + (d * 9 + i) * ng, pnb, g32, pv[i]);
+                    for (int i = 0; i < 9; ++i) pstore<BUF>(pout + (d * 9 + i) * ng, pnb, g32, pv[i]);
                 }
                 if constexpr (LS) {
                     if (valid) {
                         R *pres = a.pres + (size_t)shot * 27 * ng;
 #pragma unroll
-                        for (int i = 0; i < 9; ++i) pstore(pres + (d * 9 + i) * ng, pnb, g32, bse[d * 9 + i]);
+                        for (int i = 0; i < 9; ++i) pstore<BUF>(pres + (d * 9 + i) * ng, pnb, g32, bse[d * 9 + i]);
                     }
                 }
                 // the nine updates of a direction interleave; a scheduling
@@ -716,28 +734,28 @@ stage_kernel(StageArgs<R> a) {
         }
         const int fe = sElem[fb];
         const int fi = k / NPF, i = k - fi * NPF;
-        const int own = fb * NP + pload(a.fmask, 4u * NPF * 4u, (unsigned)(fi * NPF + i));
-        const int ft = pload(a.ftype, nel * 16u, (unsigned)fe * 4 + fi);
+        const int own = fb * NP + pload<BUF>(a.fmask, 4u * NPF * 4u, (unsigned)(fi * NPF + i));
+        const int ft = pload<BUF>(a.ftype, nel * 16u, (unsigned)fe * 4 + fi);
         // the neighbour trace is loaded unconditionally (node 0 stands in on
         // boundary faces) so the nine loads issue together
-        const int nb = pload(a.nbr, nel * (unsigned)(NFP * 4), ((unsigned)fe * 4 + fi) * NPF + i);
+        const int nb = pload<BUF>(a.nbr, nel * (unsigned)(NFP * 4), ((unsigned)fe * 4 + fi) * NPF + i);
         const unsigned gn = nb >= 0 ? (unsigned)nb : 0u;
         R dq[9];
 #pragma unroll
         for (int f = 0; f < 9; ++f) {
-            const R qn = pload(qin + f * ng, pnb, gn), qo = sQ[f * NT + own];
+            const R qn = pload<BUF>(qin + f * ng, pnb, gn), qo = sQ[f * NT + own];
             dq[f] = ft >= 0 ? qn - qo : qo;
         }
         const unsigned mb = nel * 8u * (unsigned)sizeof(R);
         R mc[5], fr[9];
 #pragma unroll
-        for (int c = 0; c < 5; ++c) mc[c] = pload(matc, mb, (unsigned)fe * 8 + c);
+        for (int c = 0; c < 5; ++c) mc[c] = pload<BUF>(matc, mb, (unsigned)fe * 8 + c);
 #pragma unroll
-        for (int c = 0; c < 9; ++c) fr[c] = pload(a.frame, nel * (unsigned)(36 * sizeof(R)), ((unsigned)fe * 4 + fi) * 9 + c);
+        for (int c = 0; c < 9; ++c) fr[c] = pload<BUF>(a.frame, nel * (unsigned)(36 * sizeof(R)), ((unsigned)fe * 4 + fi) * 9 + c);
         R fl[9], af[6];
         upwind_flux<R>(fr, mc[0], mc[1], mc[2], mc[3], mc[4], ft == -1, dq, fl, af, ATT);
         const R sgn = ft == -2 ? R(-1) : R(1);
-        const R fs = pload(a.fscale, nel * (unsigned)(NFP * sizeof(R)), (unsigned)fe * NFP + k);
+        const R fs = pload<BUF>(a.fscale, nel * (unsigned)(NFP * sizeof(R)), (unsigned)fe * NFP + k);
 #pragma unroll
         for (int f = 0; f < 9; ++f) dst[f * SFN] = fs * (sgn * fl[f]) / 2;
         if constexpr (ATT) {
@@ -758,12 +776,12 @@ stage_kernel(StageArgs<R> a) {
                 const R *av = a.anel + (size_t)shot * a.anel_stride;
 #pragma unroll
                 for (int c = 0; c < 18; ++c) {
-                    const R v = pload(thin + c * ng, pnb, g32);
+                    const R v = pload<BUF>(thin + c * ng, pnb, g32);
                     th[c] = valid ? v : R(0);
                 }
 #pragma unroll
                 for (int c = 0; c < 9; ++c) {
-                    const R v = pload(av, nel * (unsigned)(NMB * 3 * sizeof(R)), (unsigned)e * (NMB * 3) + c);
+                    const R v = pload<BUF>(av, nel * (unsigned)(NMB * 3 * sizeof(R)), (unsigned)e * (NMB * 3) + c);
                     avc[c] = valid ? v : R(0);
                 }
             }
@@ -1045,7 +1063,7 @@ stage_kernel(StageArgs<R> a) {
             } else {
 #pragma unroll
                 for (int c = 0; c < 18; ++c) {
-                    const R v = pload(thbase + c * ng, pnb, g32);
+                    const R v = pload<BUF>(thbase + c * ng, pnb, g32);
                     thb[c] = valid ? v : R(0);
                 }
             }
@@ -1056,7 +1074,7 @@ stage_kernel(StageArgs<R> a) {
         } else {
 #pragma unroll
             for (int f = 0; f < 9; ++f) {
-                const R v = pload(qbase + f * ng, pnb, g32);
+                const R v = pload<BUF>(qbase + f * ng, pnb, g32);
                 qb[f] = valid ? v : R(0);
             }
         }
@@ -1145,13 +1163,13 @@ stage_kernel(StageArgs<R> a) {
             // one guarded block of stores
             if (valid) {
 #pragma unroll
-                for (int k = 0; k < 18; ++k) pstore(thout + k * ng, pnb, g32, th[k]);
+                for (int k = 0; k < 18; ++k) pstore<BUF>(thout + k * ng, pnb, g32, th[k]);
             }
             if constexpr (LS) {
                 if (valid) {
                     R *thres = a.thres + (size_t)shot * 18 * ng;
 #pragma unroll
-                    for (int k = 0; k < 18; ++k) pstore(thres + k * ng, pnb, g32, thb[k]);
+                    for (int k = 0; k < 18; ++k) pstore<BUF>(thres + k * ng, pnb, g32, thb[k]);
                 }
             }
         }
@@ -1192,12 +1210,12 @@ stage_kernel(StageArgs<R> a) {
             R alo[9], blo[9];
 #pragma unroll
             for (int f = 0; f < 9; ++f) {
-                const R v = pload(a.qlo_in + so + f * ng, pnb, g32);
+                const R v = pload<BUF>(a.qlo_in + so + f * ng, pnb, g32);
                 alo[f] = valid ? v : R(0);
             }
 #pragma unroll
             for (int f = 0; f < 9; ++f) {
-                const R v = pload(a.qlo_base + so + f * ng, pnb_base, g32);
+                const R v = pload<BUF>(a.qlo_base + so + f * ng, pnb_base, g32);
                 blo[f] = (a.irk == 1 || !valid) ? alo[f] : v;
             }
 #pragma unroll
@@ -1205,7 +1223,7 @@ stage_kernel(StageArgs<R> a) {
                 R lo;
                 rk_combine_comp<R>(a.scheme, a.irk, qs[f], alo[f], qb[f], blo[f], rhs[f], dt, a.w13, a.w23, qnew[f],
                                    lo);
-                if (valid) { pstore(qout + f * ng, pnb, g32, qnew[f]); pstore(a.qlo_out + so + f * ng, pnb, g32, lo); }
+                if (valid) { pstore<BUF>(qout + f * ng, pnb, g32, qnew[f]); pstore<BUF>(a.qlo_out + so + f * ng, pnb, g32, lo); }
             }
         } else {
             const RkStage<R> rs = rk_stage<R>(a.scheme, a.irk, a.w13, a.w23, a.lsa, a.lsb);
@@ -1213,13 +1231,13 @@ stage_kernel(StageArgs<R> a) {
             for (int f = 0; f < 9; ++f) qnew[f] = rk_update<R, LS, sizeof(R) == 4>(rs, qs[f], qb[f], rhs[f], dt);
             if (valid) {
 #pragma unroll
-                for (int f = 0; f < 9; ++f) pstore(qout + f * ng, pnb, g32, qnew[f]);
+                for (int f = 0; f < 9; ++f) pstore<BUF>(qout + f * ng, pnb, g32, qnew[f]);
             }
             if constexpr (LS) {
                 if (valid) {
                     R *qres = a.qres + (size_t)shot * 9 * ng;
 #pragma unroll
-                    for (int f = 0; f < 9; ++f) pstore(qres + f * ng, pnb, g32, qb[f]);
+                    for (int f = 0; f < 9; ++f) pstore<BUF>(qres + f * ng, pnb, g32, qb[f]);
                 }
             }
         }
