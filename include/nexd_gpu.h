@@ -69,7 +69,10 @@ typedef struct nexd_config {
 typedef struct nexd_ctx nexd_ctx;
 
 /* Creates a context and allocates device state for the configuration.
- * Replaces the allocation block src/timeloopMod.f90:162-170. */
+ * Replaces the allocation block src/timeloopMod.f90:162-170.  The kernels
+ * address one field plane (nelem*Np doubles) and the per-element face
+ * arrays (nelem*max(4*NpF, 36) doubles) with 32-bit byte offsets: a mesh
+ * whose arrays exceed 4 GiB gets NEXD_E_UNSUPPORTED. */
 int nexd_create(const nexd_config *cfg, nexd_ctx **out);
 
 /* Reference-element operators: Dr, Ds, Dt, mass (Np,Np); lift (Np,4*NpF);
@@ -216,6 +219,8 @@ typedef struct nexd_batch_config {
 
 typedef struct nexd_batch nexd_batch;
 
+/* The same size limit as nexd_create, per shot and in the batch's
+ * precision (a field plane of nelem*Np reals below 4 GiB). */
 int nexd_batch_create(const nexd_batch_config *cfg, nexd_batch **out);
 /* same arguments and layouts as nexd_upload_operators / nexd_upload_mesh */
 int nexd_batch_upload_operators(nexd_batch *b, const double *dr, const double *ds, const double *dt,
