@@ -212,3 +212,24 @@ def test_destroy_returns_device_memory():
     print(f"[gpu batch] footprint {footprint / 2**20:.1f} MiB, lost after 3 cycles {lost / 2**20:.1f} MiB", flush=True)
     assert footprint > 0
     assert lost < 0.05 * footprint
+
+
+def test_meshes_beyond_32_bit_planes_are_refused():
+    """A mesh whose field plane exceeds 4 GiB is refused at creation with
+    NEXD_E_UNSUPPORTED (the kernels address planes with 32-bit offsets),
+    before anything is allocated."""
+    import ctypes
+    lib = capi._load_batch()
+    nelem = 10_000_000     # N = 5: 5.6e8 nodes, 4.5 GB per FP64 plane
+    cfg = capi.NexdConfig(nelem=nelem, order=5, nsrc=0, nrec=0, nt=10, timeint=2, attenuation=0, pml=0,
+                          compat=capi.COMPAT_ALL, track_displacement=0, energy=0, device=0, dt=1e-3)
+    h = ctypes.c_void_p()
+    assert lib.nexd_create(ctypes.byref(cfg), ctypes.byref(h)) == -4
+    assert b"4 GiB" in lib.nexd_last_error(None)
+    bcfg = capi.NexdBatchConfig(nelem=nelem, order=5, nshot=1, nsrc=0, nrec=0, nt=10, timeint=2, attenuation=0,
+                                pml=0, compat=capi.COMPAT_ALL, precision=capi.NEXD_FP32, shared_material=1,
+                                compensated=0, energy=0, device=0, dt=1e-3)
+    bcfg.nelem = 20_000_000    # FP32: 1.12e9 nodes, 4.5 GB per plane
+    hb = ctypes.c_void_p()
+    assert lib.nexd_batch_create(ctypes.byref(bcfg), ctypes.byref(hb)) == -4
+    assert b"4 GiB" in lib.nexd_batch_last_error(None)
