@@ -46,11 +46,17 @@ def _stack(arr_list, key):
 
 
 def pack(instances) -> list:
-    """Unions of ``instances`` (nexd_amd.batch.ShotBatch, one shot each)
-    grouped by (order, attenuation, PML, time scheme); nt must agree."""
+    """Unions of ``instances`` (nexd_amd.batch.ShotBatch, one shot each, no
+    PML) grouped by (order, attenuation, time scheme); nt must agree."""
     groups = {}
     for k, sb in enumerate(instances):
         p = sb.base
+        if p.set_pml:
+            # the reference's PML update reads the fluxes of the mesh's last
+            # element (src/timeloopMod.f90:399-405, reproduced in compat
+            # mode): in a union every instance would read the union's last
+            # element instead of its own
+            raise ValueError("instances with PML cannot be packed: their PML update reads the mesh's last element")
         key = (p.order, int(p.attenuation), int(p.set_pml), p.timeint, p.nt)
         groups.setdefault(key, []).append(k)
     nts = {key[4] for key in groups}

@@ -65,3 +65,4 @@ def test_grid_positions_first_with_corners(space):
     assert np.all(np.abs(p[27:]) <= 1.0)
     with pytest.raises(ValueError):
         surrogate.sample_positions(space, 20, 0, grid=(3, 3, 3))
+
