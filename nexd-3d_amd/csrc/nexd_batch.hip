@@ -66,6 +66,7 @@ struct Batch : BatchBase {
     int *recelem = nullptr, *rec_slot = nullptr, *rec_slot_of = nullptr;
     R *rec_int = nullptr, *u_slot = nullptr, *a_slot = nullptr, *seis = nullptr, *last_flux = nullptr;
     int *d_it = nullptr;
+    unsigned *d_done = nullptr;   // blocks of the step-end launch that finished
     int *pml_on = nullptr;                    // [shot] 1 while the shot's PML is on
     std::vector<int> pml_on_host;
     double *blk_energy = nullptr, *energy = nullptr;   // [shot][nblk_total], [shot][nt] (FP64 sums)
@@ -163,7 +164,7 @@ struct Batch : BatchBase {
         if (!rc)
             (rc = dalloc(&matc, (size_t)nmat_sets() * c.nelem * 8)) || (rc = dalloc(&emat, (size_t)nmat_sets() * c.nelem * 2)) ||
                 (rc = dalloc(&anel, (size_t)nmat_sets() * c.nelem * nexd::NMB * 3)) ||
-                (rc = dalloc(&elist_plain, c.nelem)) || (rc = dalloc(&elist_pml, c.nelem)) || (rc = dalloc(&d_it, 1)) ||
+                (rc = dalloc(&elist_plain, c.nelem)) || (rc = dalloc(&elist_pml, c.nelem)) || (rc = dalloc(&d_it, 1)) || (rc = dalloc(&d_done, 1)) ||
                 (rc = dalloc(&seis, S * 9 * c.nt * (c.nrec > 0 ? c.nrec : 1))) ||
                 (rc = dalloc(&rec_slot, c.nelem)) || (rc = dalloc(&src_head, S * c.nelem)) ||
                 (rc = dalloc(&src_next, S * (c.nsrc > 0 ? c.nsrc : 1))) ||
@@ -341,10 +342,12 @@ struct Batch : BatchBase {
 
     template <int ORDER>
     void step_end_impl(int row_offset, int advance) {
-        hipLaunchKernelGGL((nexd::step_end_kernel<ORDER, R>), dim3(1), dim3(1024), 0, stream, q[0], u_slot, a_slot,
-                           rec_int, recelem, rec_slot_of, cfg.nrec, nslot, cfg.nshot, (int)ng, cfg.nt, row_offset,
-                           advance, seis, energy_on() ? (const double *)blk_energy : (const double *)nullptr, nblk_total, energy,
-                           d_it);
+        // one block per shot and 1024 of its receiver values
+        const int chunks = cfg.nrec > 0 ? (cfg.nrec * 9 + 1023) / 1024 : 1;
+        hipLaunchKernelGGL((nexd::step_end_kernel<ORDER, R>), dim3(cfg.nshot, chunks), dim3(1024), 0, stream, q[0],
+                           u_slot, a_slot, rec_int, recelem, rec_slot_of, cfg.nrec, nslot, (int)ng, cfg.nt, row_offset,
+                           advance, seis, energy_on() ? (const double *)blk_energy : (const double *)nullptr,
+                           nblk_total, energy, d_it, d_done);
     }
 
     void step_end(int row_offset, int advance) {
@@ -465,7 +468,7 @@ struct Batch : BatchBase {
         int rc = put(seis, std::vector<R>(S * 9 * cfg.nt * (cfg.nrec > 0 ? cfg.nrec : 1), R(0)));
         if (rc) return rc;
         if (energy && (rc = put(energy, std::vector<double>(S * cfg.nt, 0.0)))) return rc;
-        if ((rc = put(d_it, std::vector<int>(1, 0)))) return rc;
+        if ((rc = put(d_it, std::vector<int>(1, 0))) || (rc = put(d_done, std::vector<unsigned>(1, 0u)))) return rc;
         step = 0;
         initialized = true;
         return 0;

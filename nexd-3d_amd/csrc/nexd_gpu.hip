@@ -72,6 +72,7 @@ struct nexd_ctx {
     double *last_flux = nullptr, *blk_energy = nullptr, *energy = nullptr;
     int nblk_total = 0;
     int *d_it = nullptr;
+    unsigned *d_done = nullptr;   // blocks of the step-end launch that finished
     int step = 0;                 // next step, 0-based
     bool have_ops = false, have_mesh = false, have_mat = false, have_att = false, have_pml = false,
          have_src = false, have_rec = false, initialized = false;
@@ -185,9 +186,11 @@ bool energy_on(const nexd_ctx *c) { return c->cfg.energy && c->cfg.pml && c->hav
 template <int ORDER>
 void launch_step_end(nexd_ctx *c, int row_offset, int advance, const double *blk_energy) {
     const double *be = energy_on(c) ? blk_energy : nullptr;
-    hipLaunchKernelGGL((nexd::step_end_kernel<ORDER, double>), dim3(1), dim3(1024), 0, c->stream, c->q[0], c->u_slot,
-                       c->a_slot, c->rec_int, c->recelem, c->rec_slot_of, c->cfg.nrec, c->nslot, 1, (int)c->nglob,
-                       c->cfg.nt, row_offset, advance, c->seis, be, c->nblk_total, c->energy, c->d_it);
+    const int chunks = c->cfg.nrec > 0 ? (c->cfg.nrec * 9 + 1023) / 1024 : 1;
+    hipLaunchKernelGGL((nexd::step_end_kernel<ORDER, double>), dim3(1, chunks), dim3(1024), 0, c->stream, c->q[0],
+                       c->u_slot, c->a_slot, c->rec_int, c->recelem, c->rec_slot_of, c->cfg.nrec, c->nslot,
+                       (int)c->nglob, c->cfg.nt, row_offset, advance, c->seis, be, c->nblk_total, c->energy, c->d_it,
+                       c->d_done);
 }
 
 void dispatch_step_end(nexd_ctx *c, int row_offset, int advance, const double *blk_energy) {
@@ -420,7 +423,7 @@ int nexd_create(const nexd_config *cfg, nexd_ctx **out) {
     }
     if (!rc)
         (rc = dalloc(c, &c->elist_plain, cfg->nelem)) || (rc = dalloc(c, &c->elist_pml, cfg->nelem)) ||
-            (rc = dalloc(c, &c->d_it, 1)) ||
+            (rc = dalloc(c, &c->d_it, 1)) || (rc = dalloc(c, &c->d_done, 1)) ||
             (rc = dalloc(c, &c->seis, (size_t)9 * cfg->nt * (cfg->nrec > 0 ? cfg->nrec : 1))) ||
             (rc = dalloc(c, &c->energy, cfg->nt)) || (rc = dalloc(c, &c->rec_slot, cfg->nelem)) ||
             (rc = dalloc(c, &c->src_head, cfg->nelem)) || (rc = dalloc(c, &c->blk_energy, (size_t)cfg->nelem + 64));
@@ -621,6 +624,7 @@ int nexd_reset(nexd_ctx *c) {
     HIPCHK(c, hipMemcpy(c->energy, ze.data(), sizeof(double) * ze.size(), hipMemcpyHostToDevice));
     int zero = 0;
     HIPCHK(c, hipMemcpy(c->d_it, &zero, sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_done, &zero, sizeof(unsigned), hipMemcpyHostToDevice));
     c->step = 0;
     c->initialized = true;
     return NEXD_OK;

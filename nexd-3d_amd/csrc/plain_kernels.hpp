@@ -1424,12 +1424,21 @@ __global__ void last_flux_kernel(const R *qin, const R *pin, const R *pmlc, cons
 
 // Receiver sampling (src/timeloopMod.f90:358-373) of row `row_offset + *d_it`
 // for every shot, each shot's energy of the step (sum of the stage kernels'
-// block partials in a fixed order), and the step-counter advance.  One block.
+// block partials in a fixed order), and the step-counter advance.
+//
+// Grid (nshot, chunks) of 1024-thread blocks: block (s, c) samples values
+// 1024 c .. 1024 c + 1023 of shot s's nrec x 9 receiver values, and block
+// (s, 0) also sums shot s's energy (the same order for any grid).  Every
+// block reads the step counter first; with more than one block the last
+// block to finish (counted in *d_done, which it puts back to 0) advances it,
+// so no block can read the next step's value.  *d_done starts at 0 and is
+// only used when the grid has more than one block.
 template <int ORDER, typename R>
 __global__ void __launch_bounds__(1024)
 step_end_kernel(const R *q, const R *u_slot, const R *a_slot, const R *rec_int, const int *recelem,
-                const int *rec_slot_of, int nrec, int nslot, int nshot, int nglob, int nt, int row_offset,
-                int advance, R *seis, const double *blk_energy, int nblk, double *energy_out, int *d_it) {
+                const int *rec_slot_of, int nrec, int nslot, int nglob, int nt, int row_offset,
+                int advance, R *seis, const double *blk_energy, int nblk, double *energy_out, int *d_it,
+                unsigned *d_done) {
     constexpr int NP = Dims<ORDER>::NP;
     __shared__ int s_it;
     __shared__ double s_red[1024];
@@ -1437,15 +1446,14 @@ step_end_kernel(const R *q, const R *u_slot, const R *a_slot, const R *rec_int, 
     __syncthreads();
     const int it = s_it;
     const int row = it + row_offset;
+    const int shot = (int)blockIdx.x;
     if (row < nt) {
-        for (int w = threadIdx.x; w < nshot * nrec * 9; w += blockDim.x) {
-            const int shot = w / (nrec * 9);
-            const int rk = w - shot * nrec * 9;
+        const R *qs = q + (size_t)shot * 9 * nglob;
+        for (int rk = blockIdx.y * blockDim.x + threadIdx.x; rk < nrec * 9; rk += gridDim.y * blockDim.x) {
             const int r = rk / 9, k = rk - r * 9;
             const int e = recelem[r];
             const size_t so = (size_t)shot * nslot + rec_slot_of[r];
             const R *wt = rec_int + (size_t)r * NP;
-            const R *qs = q + (size_t)shot * 9 * nglob;
             R acc = 0;
             for (int jn = 0; jn < NP; ++jn) {
                 R v;
@@ -1457,35 +1465,45 @@ step_end_kernel(const R *q, const R *u_slot, const R *a_slot, const R *rec_int, 
             seis[(((size_t)shot * 9 + k) * nt + row) * nrec + r] = acc;
         }
     }
-    if (advance && blk_energy) {
-        // energy_out[shot][nt]: each shot's partials [shot][nblk] in a fixed order
-        for (int shot = 0; shot < nshot; ++shot) {
-            const double *pb = blk_energy + (size_t)shot * nblk;
-            // eight independent partial sums, so eight loads are in flight per
-            // round trip; combined in a fixed order
-            constexpr int U = 8;
-            double acc[U];
+    if (advance && blk_energy && blockIdx.y == 0) {
+        // energy_out[shot][nt]: the shot's partials [shot][nblk] in a fixed order
+        const double *pb = blk_energy + (size_t)shot * nblk;
+        // eight independent partial sums, so eight loads are in flight per
+        // round trip; combined in a fixed order
+        constexpr int U = 8;
+        double acc[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) acc[u] = 0;
-            int b = threadIdx.x;
-            const int stride = blockDim.x;
-            for (; b + (U - 1) * stride < nblk; b += U * stride) {
+        for (int u = 0; u < U; ++u) acc[u] = 0;
+        int b = threadIdx.x;
+        const int stride = blockDim.x;
+        for (; b + (U - 1) * stride < nblk; b += U * stride) {
 #pragma unroll
-                for (int u = 0; u < U; ++u) acc[u] += pb[b + u * stride];
-            }
-            for (; b < nblk; b += stride) acc[0] += pb[b];
-            s_red[threadIdx.x] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
-            __syncthreads();
-            for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-                if ((int)threadIdx.x < s) s_red[threadIdx.x] += s_red[threadIdx.x + s];
-                __syncthreads();
-            }
-            if (threadIdx.x == 0 && it < nt) energy_out[(size_t)shot * nt + it] = s_red[0];
+            for (int u = 0; u < U; ++u) acc[u] += pb[b + u * stride];
+        }
+        for (; b < nblk; b += stride) acc[0] += pb[b];
+        s_red[threadIdx.x] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) s_red[threadIdx.x] += s_red[threadIdx.x + s];
             __syncthreads();
         }
+        if (threadIdx.x == 0 && it < nt) energy_out[(size_t)shot * nt + it] = s_red[0];
     }
+    if (!advance) return;
     __syncthreads();
-    if (advance && threadIdx.x == 0) *d_it = it + 1;
+    if (threadIdx.x != 0) return;
+    const unsigned nblocks = gridDim.x * gridDim.y;
+    if (nblocks == 1) {
+        *d_it = it + 1;
+        return;
+    }
+    // this block's read of the counter is done (before the barrier); the
+    // last arrival advances it
+    __threadfence();
+    if (atomicAdd(d_done, 1u) == nblocks - 1) {
+        atomicExch(d_done, 0u);
+        *d_it = it + 1;
+    }
 }
 
 }  // namespace nexd

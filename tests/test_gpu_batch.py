@@ -180,6 +180,30 @@ def test_batch_energy_and_per_shot_pml_switch(layered_shots):
         assert err <= 1e-4
 
 
+def test_many_receivers_span_several_step_end_blocks():
+    """121 receivers, so one shot's 1,089 receiver values take two blocks
+    of the step-end launch: the FP64 batch (2 shots x 2 blocks, the step
+    counter advanced by the last block) and each shot's single-shot run (1 x
+    2 blocks) against the oracle, seismograms and energy <= 1e-10."""
+    from oracle import oracle
+    sb = batch.layered_shots(2, nx=4, ny=4, nz=3, order=3, nt=120, pml_cells=1, nrec_side=11,
+                             full_source_arrays=True)
+    nrec = sb.base.rec_xyz.shape[0]
+    print(f"[gpu batch] {nrec} receivers, {sb.nshot} shots", flush=True)
+    assert nrec * 9 > 1024
+    with capi.BatchContext(sb, precision=capi.NEXD_FP64, energy=True) as b:
+        b.reset()
+        b.run(1, 50)
+        b.run(51, sb.base.nt)
+        got = [(b.seismograms(i), b.energy(i)) for i in range(sb.nshot)]
+    for i, s in enumerate(sb.shots):
+        so, eo, _ = oracle.run(layout.reference_arrays(s), compat=3)
+        assert np.all(eo > 0)
+        assert np.max(rel_l2(got[i][0], so)) <= 1e-10
+        assert float(np.max(np.abs(got[i][1] - eo) / eo)) <= 1e-10
+        assert np.max(rel_l2(fp64_single(s), so)) <= 1e-10
+
+
 def test_destroy_returns_device_memory():
     """A batch with PML that has run gives all its device memory back when
     destroyed: three create/run/destroy cycles lose less than 5 % of one
