@@ -180,6 +180,8 @@ struct Batch : BatchBase {
         have_rec = c.nrec == 0;
         std::vector<int> minus(S * c.nelem, -1);
         if ((rc = put(src_head, minus)) || (rc = put(rec_slot, std::vector<int>(c.nelem, -1)))) return rc;
+        // the step-end launch's block counter starts at 0 (each launch leaves it at 0)
+        if ((rc = put(d_done, std::vector<unsigned>(1, 0u)))) return rc;
         pml_on_host.assign(S, 1);
         if (c.pml && (rc = put(pml_on, pml_on_host))) return rc;
         return rebuild_lists();

@@ -433,6 +433,10 @@ int nexd_create(const nexd_config *cfg, nexd_ctx **out) {
     // defaults: no sources, no receivers
     std::vector<int> minus(cfg->nelem, -1);
     if ((rc = upload(c, &c->src_head, minus)) || (rc = upload(c, &c->rec_slot, minus))) return bail(rc);
+    // the step-end launch's block counter starts at 0 (each launch leaves it at 0)
+    const unsigned zero_done = 0;
+    if (hipMemcpy(c->d_done, &zero_done, sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(c, NEXD_E_HIP, "hipMemcpy failed"));
     if (!cfg->attenuation) c->have_att = true;
     if (!cfg->pml) c->have_pml = false;
     if (cfg->nsrc == 0) c->have_src = true;
