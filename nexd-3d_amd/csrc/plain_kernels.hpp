@@ -453,6 +453,13 @@ struct Shape {
     }
     static constexpr int EB = pick_eb();
     static constexpr int NT = EB * NP;
+    // FP32 PML shapes whose node threads fit in the first three waves run
+    // the face term on the fourth wave while the node threads update the
+    // PML fields (the fourth wave has no node of its own); the last
+    // element's staged fluxes then live in operator 0's flux rows, which
+    // are written only after that update
+    static constexpr bool FW = KC && NT <= 3 * 64 && NTHR == 4 * 64 && EB * NFP <= 4 * 64 && EB * 5 <= 64 &&
+                               27 * NP + EB * (4 * 9 + 5) <= 9 * NT;
     static constexpr int NCT = EB * NCOL;
     static constexpr int LDS = lds_reals(EB);
     static constexpr int OFF_F = off_face(EB);
@@ -552,7 +559,7 @@ stage_kernel(StageArgs<R> a) {
     // free until the face phase; they are loaded together with the element
     // id, so their round trip overlaps that one instead of following the
     // phase-1 batch.
-    R *sLast = lds + S::OFF_F;            // [27][NP] during phase 1
+    R *sLast = S::FW ? sC : lds + S::OFF_F;   // [27][NP] during phase 1
     static_assert(!PML || 27 * NP <= S::face_reals(EB), "last-element fluxes must fit the face region");
     constexpr int NLS = PML ? (27 * NP + NTHR - 1) / NTHR : 1;
     R lst[NLS];
@@ -591,6 +598,73 @@ stage_kernel(StageArgs<R> a) {
     // first source of the element and its receiver slot, loaded with the
     // first batch instead of at the assembly (read only where valid)
     int src_first = -1, rec_slot_e = -1;
+
+    // The face wave (FW shapes: the fourth wave, which has no node) makes
+    // the first round trip of the face term before phase 1: the element ids
+    // of its face nodes, then their neighbour indices, face types, fscale
+    // and own nodes (staged in operator 1's flux rows) and the block's face
+    // frames and materials (in operator 0's flux rows, after the last
+    // element's fluxes).  Those rows are written only after the PML update,
+    // and nothing of this is held in registers across phase 1.
+    constexpr int NFN = EB * NFP, FIT = S::FW ? (NFN + 63) / 64 : 1;
+    constexpr int NFR = EB * 4 * 9, FRL = S::FW ? (NFR + 63) / 64 : 1;
+    static_assert(!S::FW || 4 * NFN <= 9 * NT, "face-node data must fit operator 1's flux rows");
+    const bool fwave = S::FW && (t >> 6) == S::NW - 1;
+    R *sFr = sC + 27 * NP;                    // [EB*4][9] face frames (FW shapes)
+    R *sMc = sFr + NFR;                       // [EB][5] materials
+    R *sFn = sC + 12 * NT;                    // [4][NFN] neighbour node, face type, own node (bits), fscale
+    if constexpr (S::FW) {
+        if (fwave) {
+            const int l = t & 63;
+            int fel[EB];
+#pragma unroll
+            for (int b = 0; b < EB; ++b) fel[b] = eblk * EB + b < a.nlist ? a.elist[eblk * EB + b] : 0;
+            int fnb[FIT], fty[FIT], fown[FIT];
+            R ffs[FIT], frv[FRL], mcv;
+#pragma unroll
+            for (int i = 0; i < FIT; ++i) {
+                const int q = l + 64 * i, qq = q < NFN ? q : 0;
+                const int fb = qq / NFP, k = qq - fb * NFP, fi = k / NPF, ii = k - fi * NPF;
+                int fe = fel[0];
+#pragma unroll
+                for (int b = 1; b < EB; ++b) fe = fb == b ? fel[b] : fe;
+                fown[i] = fb * NP + pload<BUF>(a.fmask, 4u * NPF * 4u, (unsigned)(fi * NPF + ii));
+                fty[i] = pload<BUF>(a.ftype, nel * 16u, (unsigned)fe * 4 + fi);
+                fnb[i] = pload<BUF>(a.nbr, nel * (unsigned)(NFP * 4), ((unsigned)fe * 4 + fi) * NPF + ii);
+                ffs[i] = pload<BUF>(a.fscale, nel * (unsigned)(NFP * sizeof(R)), (unsigned)fe * NFP + k);
+            }
+#pragma unroll
+            for (int r = 0; r < FRL; ++r) {
+                const int v = l + 64 * r, vv = v < NFR ? v : 0;
+                const int face = vv / 9, c = vv - face * 9, fb = face / 4;
+                int fe = fel[0];
+#pragma unroll
+                for (int b = 1; b < EB; ++b) fe = fb == b ? fel[b] : fe;
+                frv[r] = pload<BUF>(a.frame, nel * (unsigned)(36 * sizeof(R)), ((unsigned)fe * 4 + face % 4) * 9 + c);
+            }
+            {
+                const int ll = l < EB * 5 ? l : 0, fb = ll / 5, c = ll - fb * 5;
+                int fe = fel[0];
+#pragma unroll
+                for (int b = 1; b < EB; ++b) fe = fb == b ? fel[b] : fe;
+                mcv = pload<BUF>(matc, nel * 8u * (unsigned)sizeof(R), (unsigned)fe * 8 + c);
+            }
+#pragma unroll
+            for (int i = 0; i < FIT; ++i) {
+                const int q = l + 64 * i;
+                if (q < NFN) {
+                    sFn[q] = __builtin_bit_cast(R, fnb[i]);
+                    sFn[NFN + q] = __builtin_bit_cast(R, fty[i]);
+                    sFn[2 * NFN + q] = __builtin_bit_cast(R, fown[i]);
+                    sFn[3 * NFN + q] = ffs[i];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < FRL; ++r)
+                if (l + 64 * r < NFR) sFr[l + 64 * r] = frv[r];
+            if (l < EB * 5) sMc[l] = mcv;
+        }
+    }
 
     // Phase 1 issues every load of the node threads in one batch: q, metric
     // and material and (PML) the coefficients and all 27 auxiliaries and RK
@@ -639,6 +713,62 @@ stage_kernel(StageArgs<R> a) {
         for (int k = 0; k < 9; ++k) (KC ? sC + ((k / 3) * 12 + 9 + k % 3) * NT : sM + k * NT)[t] = mv[k];
         rho = m2;
         ef0 = -(m3 + 2 * m4); ef1 = -m3; ef2 = -m4; ef3 = R(-1.0) / rho;
+    }
+    if constexpr (S::FW) __syncthreads();   // q is in LDS for the face wave
+    R cf[KC ? 27 : 1];
+    if (fwave) {
+        if constexpr (S::FW) {
+            // ---- face term on the face wave, while the node threads update
+            // the PML fields: the upwind flux of every face node of the
+            // block into sF, as the face phase below computes it; the
+            // second round trip, the neighbour traces
+            const int l = t & 63;
+            int fnb[FIT], fty[FIT], fown[FIT];
+            R ffs[FIT];
+#pragma unroll
+            for (int i = 0; i < FIT; ++i) {
+                const int q = l + 64 * i, qq = q < NFN ? q : 0;
+                fnb[i] = __builtin_bit_cast(int, sFn[qq]);
+                fty[i] = __builtin_bit_cast(int, sFn[NFN + qq]);
+                fown[i] = __builtin_bit_cast(int, sFn[2 * NFN + qq]);
+                ffs[i] = sFn[3 * NFN + qq];
+            }
+            R fqn[FIT][9];
+#pragma unroll
+            for (int i = 0; i < FIT; ++i) {
+                const unsigned gn = fnb[i] >= 0 ? (unsigned)fnb[i] : 0u;
+#pragma unroll
+                for (int f = 0; f < 9; ++f) fqn[i][f] = pload<BUF>(qin + f * ng, pnb, gn);
+            }
+#pragma unroll
+            for (int i = 0; i < FIT; ++i) {
+                const int q = l + 64 * i;
+                if (q < NFN) {
+                    const int fb = q / NFP, k = q - fb * NFP, fi = k / NPF;
+                    const bool fvalid = eblk * EB + fb < a.nlist;
+                    R dq[9], fr[9];
+#pragma unroll
+                    for (int f = 0; f < 9; ++f) {
+                        const R qo = sQ[f * NT + fown[i]];
+                        dq[f] = fty[i] >= 0 ? fqn[i][f] - qo : qo;
+                    }
+#pragma unroll
+                    for (int c = 0; c < 9; ++c) fr[c] = sFr[(fb * 4 + fi) * 9 + c];
+                    const R *mc = sMc + fb * 5;
+                    R fl[9], af[6];
+                    upwind_flux<R>(fr, mc[0], mc[1], mc[2], mc[3], mc[4], fty[i] == -1, dq, fl, af, ATT);
+                    const R sgn = fty[i] == -2 ? R(-1) : R(1);
+                    R *dst = sF + fb * NFC * SFN + k;
+#pragma unroll
+                    for (int f = 0; f < 9; ++f) dst[f * SFN] = fvalid ? ffs[i] * (sgn * fl[f]) / 2 : R(0);
+                    if constexpr (ATT) {
+#pragma unroll
+                        for (int f = 0; f < 6; ++f) dst[(9 + f) * SFN] = fvalid ? ffs[i] * (sgn * af[f]) / 2 : R(0);
+                    }
+                }
+            }
+        }
+    } else if (node_thr) {
         if constexpr (PML) {
             // modified fluxes (src/timeloopMod.f90:399-405) and the ADE update
             // of f',g',h' (src/rungekuttaMod.f90:52-63); the update needs only
@@ -656,7 +786,6 @@ stage_kernel(StageArgs<R> a) {
             // F_i + o_y G_i + o_z H_i, o = r, s, t with the node's metric
             // (mv = rx ry rz sx sy sz tx ty tz)
             int col = 0;
-            R cf[KC ? 27 : 1];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 R pv[9];
@@ -705,10 +834,15 @@ stage_kernel(StageArgs<R> a) {
                 // fence per direction keeps the working set in registers
                 __builtin_amdgcn_sched_barrier(0);
             }
-            if constexpr (KC) {
+        }
+    }
+    // the staged fluxes (and the face wave's frames) are read: the flux rows
+    // may be written
+    if constexpr (S::FW) __syncthreads();
+    if constexpr (KC) {
+        if (node_thr) {
 #pragma unroll
-                for (int c = 0; c < 27; ++c) sC[((c / 9) * 12 + c % 9) * NT + t] = cf[c];
-            }
+            for (int c = 0; c < 27; ++c) sC[((c / 9) * 12 + c % 9) * NT + t] = cf[c];
         }
     }
     __syncthreads();
@@ -720,8 +854,9 @@ stage_kernel(StageArgs<R> a) {
 #pragma unroll
     for (int ks = 0; ks < S::KSL; ++ks) lfa[ks] = lfr[ks * 64];
 
-    // ---- face term: upwind flux per face node into sF ----------------------
-    for (int it = t; it < EB * NFP; it += NTHR) {
+    // ---- face term: upwind flux per face node into sF (done above by the
+    // face wave in the FW shapes) ---------------------------------------------
+    for (int it = t; it < (S::FW ? 0 : EB * NFP); it += NTHR) {
         const int fb = it / NFP, k = it - fb * NFP;
         const int fidx = eblk * EB + fb;
         // value f of this face node at dst[f * SFN]: consecutive threads
