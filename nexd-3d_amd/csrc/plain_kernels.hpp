@@ -556,9 +556,10 @@ stage_kernel(StageArgs<R> a) {
     const bool valid = node_thr && idx < a.nlist;
     // The last element's fluxes (PML, compat), the same for every element,
     // are staged once per block in the face-value region of LDS, which is
-    // free until the face phase; they are loaded together with the element
-    // id, so their round trip overlaps that one instead of following the
-    // phase-1 batch.
+    // free until the face phase (in the FW shapes, where the face wave
+    // writes that region during the PML update, in operator 0's flux rows);
+    // they are loaded together with the element id, so their round trip
+    // overlaps that one instead of following the phase-1 batch.
     R *sLast = S::FW ? sC : lds + S::OFF_F;   // [27][NP] during phase 1
     static_assert(!PML || 27 * NP <= S::face_reals(EB), "last-element fluxes must fit the face region");
     constexpr int NLS = PML ? (27 * NP + NTHR - 1) / NTHR : 1;
